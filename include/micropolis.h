@@ -1,0 +1,191 @@
+/*
+ * micropolis_amd -- C ABI of the MI355X (gfx950) REYES hot path.
+ *
+ * This is the drop-in boundary for the OpenCL backend of the micropolis
+ * renderer.  Each entry point names the reference interface it replaces.
+ * All arguments are plain pointers and sizes; nothing throws across the ABI;
+ * every call returns MP_OK (0) or a negative MP_E* code and records a
+ * message retrievable with mp_last_error().
+ *
+ * Threading: a context is single-threaded (like Reyes::RendererCL, which is
+ * driven by one host thread through one command queue).  All device work of
+ * a context is ordered on one HIP stream; only mp_finish() and mp_read_*()
+ * block the caller.
+ */
+#ifndef MICROPOLIS_AMD_H
+#define MICROPOLIS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MP_OK           0
+#define MP_E_INVALID   -1   /* bad argument / unsupported option          */
+#define MP_E_HIP       -2   /* HIP runtime error                          */
+#define MP_E_NOMEM     -3   /* device or host allocation failed           */
+#define MP_E_STATE     -4   /* call out of order (e.g. draw before prepare) */
+#define MP_E_NOTFOUND  -5   /* unknown patch handle                       */
+
+/* bound_n_split_method (src/Reyes/config.xml:34-39; BALANCED from
+ * kernels/bound_n_split_balanced.cl, which the reference does not wire) */
+enum mp_bns_method {
+    MP_BNS_CPU      = 0,
+    MP_BNS_BOUNDED  = 1,
+    MP_BNS_LOCAL    = 2,
+    MP_BNS_BREADTH  = 3,
+    MP_BNS_BALANCED = 4
+};
+
+/* PatchType (src/Reyes/PatchType.h) */
+enum mp_patch_type {
+    MP_BEZIER  = 0,   /* 16 control points per patch */
+    MP_GREGORY = 1    /* 20 control points per patch */
+};
+
+/* Hot-path options of reyes.options / src/Reyes/config.xml:62-124.  Names
+ * follow the reference option names. */
+typedef struct mp_config {
+    int32_t window_width, window_height;   /* window_size                     */
+    float   clear_color[4];                /* clear_color                     */
+    int32_t reyes_patch_size;              /* N: 8 or 16                      */
+    float   bound_n_split_limit;           /* screen length limit (pixels)    */
+    float   cull_ribbon;
+    int32_t backface_culling;
+    int32_t bound_n_split_method;          /* enum mp_bns_method              */
+    int32_t bound_sample_rate;             /* must be 3 on the GPU path       */
+    int32_t max_split_depth;               /* <= 23                           */
+    int32_t reyes_patches_per_pass;        /* pass size (BREADTH truncation,  */
+                                           /* BOUNDED batch, LOCAL out cap)   */
+    int32_t framebuffer_tile_size;         /* must be 8                       */
+    int32_t subpixel_bits;                 /* PXLCOORD_SHIFT                  */
+    int32_t dice_group_width;              /* accepted, unused on gfx950      */
+    int32_t bns_pipeline_length;           /* CPU method transfer ring        */
+    int32_t local_bns_work_groups;         /* LOCAL/BALANCED work-groups;     */
+                                           /* 0 = fill the device             */
+    int32_t displacement;                  /* 0 off, 1 3D value noise         */
+    float   displacement_amplitude;
+    float   displacement_frequency;
+    uint32_t displacement_seed;
+    int32_t displacement_octaves;
+    int32_t dummy_render;                  /* split only, no raster           */
+} mp_config;
+
+/* Reyes::Projection constructor arguments (src/base/Projection.h:44) */
+typedef struct mp_projection {
+    float   fovy_deg;
+    float   near_plane;
+    int32_t width, height;
+} mp_projection;
+
+/* Statistics of the last finished frame (src/base/Statistics.cpp:160-218). */
+typedef struct mp_stats {
+    int64_t patches_per_frame;        /* DRAW ranges emitted by bound&split    */
+    int64_t diced_ranges;             /* ranges rasterised (BREADTH truncates) */
+    int64_t dropped_ranges;           /* BREADTH ranges beyond the pass size   */
+    int64_t bounded_patches;          /* bound() evaluations                   */
+    int64_t micropolygons;            /* reyes_patch_size^2 * diced_ranges     */
+    int64_t pass_count;               /* render passes                         */
+    int64_t max_patches;              /* largest in-flight split level/stack   */
+    int64_t slow_blocks;              /* raster blocks on the wide-int path    */
+    int64_t overflow_relaunches;      /* persistent split relaunches           */
+    double  frame_ms;                 /* mp_prepare..mp_finish device time     */
+} mp_stats;
+
+typedef struct mp_context mp_context;
+
+/* Fill *cfg with the values of the reference's reyes.options. */
+int mp_config_defaults(mp_config* cfg);
+
+/* Reyes::RendererCL::RendererCL (src/Reyes/RendererCL.cpp:21-106).
+ * Returns NULL on failure; mp_create_error() then holds the reason. */
+mp_context* mp_create(const mp_config* cfg);
+const char* mp_create_error(void);
+
+void mp_destroy(mp_context* ctx);
+
+const char* mp_last_error(const mp_context* ctx);
+
+/* Renderer::are_patches_loaded (src/Reyes/Renderer.h:36) */
+int mp_are_patches_loaded(mp_context* ctx, uint64_t handle);
+
+/* Renderer::load_patches / PatchIndex::load_patches
+ * (src/Reyes/Renderer.h:37, src/Reyes/PatchIndex.cpp:57-100):
+ * xyz holds n_points * 3 floats (16 points per Bezier, 20 per Gregory). */
+int mp_load_patches(mp_context* ctx, uint64_t handle, const float* xyz,
+                    size_t n_points, int32_t patch_type);
+
+/* PatchIndex::delete_patches (src/Reyes/PatchIndex.cpp:103-108) */
+int mp_unload_patches(mp_context* ctx, uint64_t handle);
+
+/* Renderer::prepare (src/Reyes/RendererCL.cpp:115-135): starts a frame. */
+int mp_prepare(mp_context* ctx);
+
+/* Renderer::draw_patches (src/Reyes/RendererCL.cpp:170-200).  mv is the
+ * column-major modelview (Scene.cpp:190), color the diffuse colour. */
+int mp_draw_patches(mp_context* ctx, uint64_t handle, const float mv[16],
+                    const mp_projection* projection, const float color[4]);
+
+/* Renderer::finish (src/Reyes/RendererCL.cpp:138-154): completes the frame
+ * (split, dice, shade, sample, resolve) and waits for it. */
+int mp_finish(mp_context* ctx);
+
+/* Framebuffer readback (src/Reyes/Framebuffer.cpp:107-124, display
+ * transform shaders/tex_draw.frag:25-37).  Row-major W*H, row 0 = bottom
+ * (OpenGL window convention).  Any pointer may be NULL.
+ *   rgba8  : display colour, gamma 1/2.2, alpha 255
+ *   depth  : FP32 depth of the visible sample (1.0 where empty)
+ *   ids    : visible micropolygon id (0xFFFFFFFF where empty);
+ *            id = range_index * N*N + v*N + u, range_index in canonical
+ *            (object, patch id, umin, vmin) order
+ *   rgba32 : linear RGBA32F framebuffer value (the reference's buffer) */
+int mp_read_framebuffer(mp_context* ctx, uint8_t* rgba8, float* depth,
+                        uint32_t* ids, float* rgba32);
+
+/* Debug tap: DRAW ranges of the last frame in canonical order, as
+ * (pid, object, depth, umin, vmin, umax, vmax) records; *count receives the
+ * number of ranges.  Pass records == NULL to query the count. */
+typedef struct mp_range_record {
+    int32_t pid;
+    int32_t object;
+    int32_t depth;
+    float   umin, vmin, umax, vmax;
+} mp_range_record;
+int mp_read_ranges(mp_context* ctx, mp_range_record* records, int64_t capacity,
+                   int64_t* count);
+
+/* Debug tap: per-pass DRAW counts of the last frame (BREADTH: per split
+ * level; BOUNDED: per batch; LOCAL/BALANCED: per persistent launch). */
+int mp_read_pass_counts(mp_context* ctx, int32_t* counts, int32_t capacity,
+                        int32_t* n_passes);
+
+/* Statistics (src/base/Statistics.cpp:160-218) of the last frame. */
+int mp_get_stats(mp_context* ctx, mp_stats* stats);
+
+/* ---- batching: many independent views/scenes rendered by one set of
+ * persistent launches (no reference counterpart; the plain path is the
+ * 1-view case of the same machinery). ---- */
+
+/* Start a batch of n_views views, each with its own framebuffer of
+ * projections[v].width x projections[v].height. */
+int mp_batch_begin(mp_context* ctx, int32_t n_views, const mp_projection* projections);
+/* Add an object to view `view` (same arguments as mp_draw_patches). */
+int mp_batch_draw(mp_context* ctx, int32_t view, uint64_t handle, const float mv[16],
+                  const float color[4]);
+/* Render every view of the batch and wait. */
+int mp_batch_finish(mp_context* ctx);
+/* Read back view `view` of the last batch (same layout as mp_read_framebuffer). */
+int mp_batch_read(mp_context* ctx, int32_t view, uint8_t* rgba8, float* depth,
+                  uint32_t* ids, float* rgba32);
+
+/* ---- device-resident benchmarking: replay the last frame or batch n times
+ * on the device (inputs resident), returning the mean device ms per replay
+ * measured with HIP events around the replays. ---- */
+int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* ms_per_frame);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

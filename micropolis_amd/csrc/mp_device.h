@@ -1,0 +1,354 @@
+// Device math of the REYES hot path for gfx950.
+//
+// Every floating-point expression follows the operation order written in the
+// reference kernels (kernels/utility.h, kernels/bound_n_split.h,
+// kernels/dice.cl, kernels/reyes.cl); the library is built with
+// -ffp-contract=off and the default correctly rounded f32 '/' and sqrt, so
+// results are bit-identical to an IEEE-strict scalar evaluation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mp_types.h"
+
+#define MP_DEV __device__ __forceinline__
+
+namespace mp {
+
+// OpenCL min(x,y): y if y < x else x.  OpenCL max(x,y): y if x < y else x.
+MP_DEV float cl_min(float x, float y) { return (y < x) ? y : x; }
+MP_DEV float cl_max(float x, float y) { return (x < y) ? y : x; }
+
+MP_DEV int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+MP_DEV int32_t wsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+MP_DEV int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+
+// float -> int32: NaN -> 0, saturating, else truncation toward zero.
+MP_DEV int32_t f2i_sat(float x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return (int32_t)0x80000000u;
+    return (int32_t)x;
+}
+
+MP_DEV float dot3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    return (ax * bx + ay * by) + az * bz;
+}
+
+// column-major 4x4 times vector, rows summed left to right
+MP_DEV void mat4_mul(const float* __restrict__ m, const float v[4], float o[4])
+{
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        o[i] = ((m[0 + i] * v[0] + m[4 + i] * v[1]) + m[8 + i] * v[2]) + m[12 + i] * v[3];
+}
+
+MP_DEV void bern_u(float t, float s, float a[4])
+{
+    a[0] = (s * s) * s;
+    a[1] = ((3.0f * s) * s) * t;
+    a[2] = ((3.0f * s) * t) * t;
+    a[3] = (t * t) * t;
+}
+MP_DEV void bern_v(float t, float s, float b[4])
+{
+    b[0] = (s * s) * s;
+    b[1] = ((3.0f * t) * s) * s;
+    b[2] = ((3.0f * t) * t) * s;
+    b[3] = (t * t) * t;
+}
+MP_DEV void dbern(float t, float s, float d[4])
+{
+    d[0] = -3.0f * (s * s);
+    d[1] = 3.0f * (s * (s - 2.0f * t));
+    d[2] = 3.0f * (t * (2.0f * s - t));
+    d[3] = 3.0f * (t * t);
+}
+
+MP_DEV void acc4(float p[4], float w, const float4 P)
+{
+    p[0] = p[0] + w * P.x;
+    p[1] = p[1] + w * P.y;
+    p[2] = p[2] + w * P.z;
+    p[3] = p[3] + w * P.w;
+}
+
+// Bicubic Bezier (utility.h:41-72); cp = 16 control points (xyz, w=1).
+MP_DEV void eval_bezier(const float4* __restrict__ cp, float u, float v, float p[4])
+{
+    float su = 1.0f - u, sv = 1.0f - v;
+    float a[4], b[4];
+    bern_u(u, su, a);
+    bern_v(v, sv, b);
+    p[0] = p[1] = p[2] = p[3] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            acc4(p, a[i] * b[j], cp[4 * i + j]);
+}
+
+MP_DEV float nz(float x) { return (x != x) ? 0.0f : x; }
+
+MP_DEV float4 greg_f(float nx, float ny, float nz_, float nw, float d)
+{
+    return make_float4(nz(nx / d), nz(ny / d), nz(nz_ / d), nz(nw / d));
+}
+
+// 20-point Gregory patch (utility.h:75-126).
+MP_DEV void eval_gregory(const float4* __restrict__ P, float u, float v, float p[4])
+{
+    const float omu = 1.0f - u, omv = 1.0f - v;
+    const float d0 = u + v, d1 = (1.0f - u) + v, d2 = (2.0f - u) - v, d3 = (1.0f + u) - v;
+    const float4 a12 = P[12], a16 = P[16], a17 = P[17], a13 = P[13];
+    const float4 a14 = P[14], a18 = P[18], a19 = P[19], a15 = P[15];
+    float4 F0 = greg_f(u * a12.x + v * a16.x, u * a12.y + v * a16.y, u * a12.z + v * a16.z,
+                       u * a12.w + v * a16.w, d0);
+    float4 F1 = greg_f(omu * a17.x + v * a13.x, omu * a17.y + v * a13.y, omu * a17.z + v * a13.z,
+                       omu * a17.w + v * a13.w, d1);
+    float4 F2 = greg_f(omu * a14.x + omv * a18.x, omu * a14.y + omv * a18.y,
+                       omu * a14.z + omv * a18.z, omu * a14.w + omv * a18.w, d2);
+    float4 F3 = greg_f(u * a19.x + omv * a15.x, u * a19.y + omv * a15.y, u * a19.z + omv * a15.z,
+                       u * a19.w + omv * a15.w, d3);
+    float su = 1.0f - u, sv = 1.0f - v;
+    float a[4], b[4];
+    bern_u(u, su, a);
+    bern_v(v, sv, b);
+    p[0] = p[1] = p[2] = p[3] = 0.0f;
+    acc4(p, a[0] * b[0], P[0]);  acc4(p, a[0] * b[1], P[4]);
+    acc4(p, a[0] * b[2], P[9]);  acc4(p, a[0] * b[3], P[1]);
+    acc4(p, a[1] * b[0], P[8]);  acc4(p, a[1] * b[1], F0);
+    acc4(p, a[1] * b[2], F1);    acc4(p, a[1] * b[3], P[5]);
+    acc4(p, a[2] * b[0], P[7]);  acc4(p, a[2] * b[1], F3);
+    acc4(p, a[2] * b[2], F2);    acc4(p, a[2] * b[3], P[10]);
+    acc4(p, a[3] * b[0], P[3]);  acc4(p, a[3] * b[1], P[11]);
+    acc4(p, a[3] * b[2], P[6]);  acc4(p, a[3] * b[3], P[2]);
+}
+
+MP_DEV void eval_patch(const float4* __restrict__ cp, int type, float u, float v, float p[4])
+{
+    if (type == MP_TYPE_GREGORY) eval_gregory(cp, u, v, p);
+    else                         eval_bezier(cp, u, v, p);
+}
+
+// ---- 3D value noise (defined by this library; see the oracle for the
+// identical scalar statement) ----
+MP_DEV uint32_t noise_hash(int32_t x, int32_t y, int32_t z, uint32_t seed)
+{
+    uint32_t h = seed ^ 0x9e3779b9u;
+    h ^= (uint32_t)x * 0x8da6b343u;
+    h ^= (uint32_t)y * 0xd8163841u;
+    h ^= (uint32_t)z * 0xcb1ab31fu;
+    h ^= h >> 16; h *= 0x85ebca6bu;
+    h ^= h >> 13; h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+MP_DEV float lattice(int32_t x, int32_t y, int32_t z, uint32_t seed)
+{
+    return (float)(noise_hash(x, y, z, seed) >> 8) * (1.0f / 8388608.0f) - 1.0f;
+}
+MP_DEV float fade(float t) { return (t * t) * (3.0f - 2.0f * t); }
+MP_DEV float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
+
+MP_DEV float vnoise(float x, float y, float z, uint32_t seed)
+{
+    float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
+    float wx = fade(x - fx), wy = fade(y - fy), wz = fade(z - fz);
+    float c000 = lattice(ix, iy, iz, seed),         c100 = lattice(ix + 1, iy, iz, seed);
+    float c010 = lattice(ix, iy + 1, iz, seed),     c110 = lattice(ix + 1, iy + 1, iz, seed);
+    float c001 = lattice(ix, iy, iz + 1, seed),     c101 = lattice(ix + 1, iy, iz + 1, seed);
+    float c011 = lattice(ix, iy + 1, iz + 1, seed), c111 = lattice(ix + 1, iy + 1, iz + 1, seed);
+    float x00 = lerpf_(c000, c100, wx), x10 = lerpf_(c010, c110, wx);
+    float x01 = lerpf_(c001, c101, wx), x11 = lerpf_(c011, c111, wx);
+    float y0 = lerpf_(x00, x10, wy), y1 = lerpf_(x01, x11, wy);
+    return lerpf_(y0, y1, wz);
+}
+
+MP_DEV float noise3(const DevParams& prm, float x, float y, float z)
+{
+    float acc = 0.0f, amp = 1.0f, norm = 0.0f, freq = prm.disp_frequency;
+    for (int k = 0; k < prm.disp_octaves; ++k) {
+        acc = acc + amp * vnoise(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k);
+        norm = norm + amp;
+        amp = amp * 0.5f;
+        freq = freq * 2.0f;
+    }
+    return acc / norm;
+}
+
+// Displaced object-space point P + amp*noise(P)*unit(cross(dP/dv, dP/du)).
+MP_DEV void displaced_point(const float4* __restrict__ cp, int type, float u, float v,
+                            const DevParams& prm, float amp, float out[4])
+{
+    eval_patch(cp, type, u, v, out);
+    if (!prm.displacement || type != MP_TYPE_BEZIER) return;
+    float su = 1.0f - u, sv = 1.0f - v;
+    float a[4], b[4], da[4], db[4];
+    bern_u(u, su, a);
+    bern_v(v, sv, b);
+    dbern(u, su, da);
+    dbern(v, sv, db);
+    float pux = 0.f, puy = 0.f, puz = 0.f, pvx = 0.f, pvy = 0.f, pvz = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float4 P = cp[4 * i + j];
+            float wu = da[i] * b[j], wv = a[i] * db[j];
+            pux = pux + wu * P.x; puy = puy + wu * P.y; puz = puz + wu * P.z;
+            pvx = pvx + wv * P.x; pvy = pvy + wv * P.y; pvz = pvz + wv * P.z;
+        }
+    // n = cross(pv, pu)
+    float nx = pvy * puz - pvz * puy;
+    float ny = pvz * pux - pvx * puz;
+    float nz_ = pvx * puy - pvy * pux;
+    float len2 = dot3(nx, ny, nz_, nx, ny, nz_);
+    float hx = 0.f, hy = 0.f, hz = 0.f;
+    if (len2 > 0.0f) {
+        float inv = 1.0f / sqrtf(len2);
+        hx = nx * inv; hy = ny * inv; hz = nz_ * inv;
+    }
+    float d = amp * noise3(prm, out[0], out[1], out[2]);
+    out[0] = out[0] + d * hx;
+    out[1] = out[1] + d * hy;
+    out[2] = out[2] + d * hz;
+}
+
+// ---- bound() (kernels/bound_n_split.h:43-100) with RES = 3 ----
+MP_DEV int bound3(const float4* __restrict__ cp, int type, int depth, float umin, float vmin,
+                  float umax, float vmax, const float* __restrict__ mv, const DevView& P,
+                  const DevParams& prm, float split_limit)
+{
+    if (depth >= prm.max_split_depth - 1) return MP_CULL;
+    float px[3][3], py[3][3];
+    float bminx = INFINITY, bminy = INFINITY, bminz = INFINITY;
+    float bmaxx = -INFINITY, bmaxy = -INFINITY, bmaxz = -INFINITY;
+#pragma unroll
+    for (int iu = 0; iu < 3; ++iu) {
+#pragma unroll
+        for (int iv = 0; iv < 3; ++iv) {
+            float tu = (float)iu * 0.5f, tv = (float)iv * 0.5f;
+            float u = umin + (umax - umin) * tu;
+            float v = vmin + (vmax - vmin) * tv;
+            float p[4], e[4], q[4];
+            eval_patch(cp, type, u, v, p);
+            mat4_mul(mv, p, e);
+            bminx = cl_min(bminx, e[0]); bminy = cl_min(bminy, e[1]); bminz = cl_min(bminz, e[2]);
+            bmaxx = cl_max(bmaxx, e[0]); bmaxy = cl_max(bmaxy, e[1]); bmaxz = cl_max(bmaxz, e[2]);
+            mat4_mul(P.proj, e, q);
+            float rx = q[0] / q[3], ry = q[1] / q[3];
+            px[iv][iu] = P.screen[0] * rx + P.screen[2] * ry;
+            py[iv][iu] = P.screen[1] * rx + P.screen[3] * ry;
+        }
+    }
+    // outside_frustum (bound_n_split.h:10-31)
+    {
+        float n = -bmaxz;
+        float f = -bminz;
+        if (f < P.near_ || n > P.far_) return MP_CULL;
+        n = cl_max(P.near_, n);
+        float hw = (float)P.width * 0.5f, hh = (float)P.height * 0.5f;
+        float sminx = bminx / ((bminx < 0) ? n : f) * P.f[0] + hw;
+        float sminy = bminy / ((bminy < 0) ? n : f) * P.f[1] + hh;
+        float smaxx = bmaxx / ((bmaxx > 0) ? n : f) * P.f[0] + hw;
+        float smaxy = bmaxy / ((bmaxy > 0) ? n : f) * P.f[1] + hh;
+        const float rib = prm.cull_ribbon;
+        if (sminx > (float)(P.width - 1) + rib || smaxx < -rib ||
+            sminy > (float)(P.height - 1) + rib || smaxy < -rib)
+            return MP_CULL;
+    }
+    if (bminz < P.eps && bmaxz > P.near_)
+        return ((umax - umin) < (vmax - vmin)) ? MP_VSPLIT : MP_HSPLIT;
+    float hlen = 0.0f, vlen = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float h = 0.0f, v = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            float dx = px[i][j] - px[i][j + 1];
+            float dy = py[i][j] - py[i][j + 1];
+            h = h + sqrtf(dx * dx + dy * dy);
+            dx = px[j][i] - px[j + 1][i];
+            dy = py[j][i] - py[j + 1][i];
+            v = v + sqrtf(dx * dx + dy * dy);
+        }
+        hlen = cl_max(h, hlen);
+        vlen = cl_max(v, vlen);
+    }
+    if (hlen <= split_limit && vlen <= split_limit) return MP_DRAW;
+    return (hlen > vlen) ? MP_VSPLIT : MP_HSPLIT;
+}
+
+// Children of a split range (bound_n_split_breadthfirst.cl:99-113)
+MP_DEV void split_children(const RangeRec& r, int flags, RangeRec& c0, RangeRec& c1)
+{
+    float cx = (r.umin + r.umax) * 0.5f;
+    float cy = (r.vmin + r.vmax) * 0.5f;
+    c0 = r; c1 = r;
+    uint32_t od = r.obj_depth + (1u << 24);
+    c0.obj_depth = od; c1.obj_depth = od;
+    if (flags & 4) { c0.umax = cx; c1.umin = cx; }
+    else           { c0.vmax = cy; c1.vmin = cy; }
+}
+
+MP_DEV uint64_t canonical_key(const RangeRec& r)
+{
+    uint64_t u = (uint64_t)(uint32_t)(r.umin * 4194304.0f);
+    uint64_t v = (uint64_t)(uint32_t)(r.vmin * 4194304.0f);
+    return (u << 22) | v;
+}
+
+// monotone uint32 image of a float (total order consistent with <)
+MP_DEV uint32_t mono_bits(float d)
+{
+    uint32_t b = __float_as_uint(d);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+MP_DEV float unmono_bits(uint32_t m)
+{
+    uint32_t b = (m & 0x80000000u) ? (m & 0x7fffffffu) : ~m;
+    return __uint_as_float(b);
+}
+
+// inside_triangle (kernels/reyes.cl:240-265), wrapping int32 arithmetic.
+MP_DEV bool inside_tri(int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t x2, int32_t y2,
+                       float d0, float d1, float d2, int32_t tx, int32_t ty, float& depth)
+{
+    const int32_t Px[3] = { x0, x1, x2 }, Py[3] = { y0, y1, y2 };
+    int32_t Dx[3], Dy[3], O[3], V[3];
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        int j = (i + 1) % 3;
+        Dx[i] = wsub(Py[j], Py[i]);
+        Dy[i] = wsub(Px[i], Px[j]);
+        O[i] = wadd(wmul(Dx[i], Px[i]), wmul(Dy[i], Py[i]));
+        int32_t C = (Dx[i] > 0 || (Dx[i] == 0 && Dy[i] > 0)) ? -1 : 0;
+        V[i] = wsub(wadd(wmul(Dx[i], tx), wmul(Dy[i], ty)), O[i]);
+        ok = ok && (V[i] > C);
+    }
+    if (ok) {
+        float w[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int j = (k + 1) % 3;
+            int32_t den = wsub(wadd(wmul(Dx[j], Px[k]), wmul(Dy[j], Py[k])), O[j]);
+            w[k] = (float)V[j] / (float)den;
+        }
+        float d = (d0 * w[0] + d1 * w[1]) + d2 * w[2];
+        if (d < depth) depth = d;
+    }
+    return ok;
+}
+
+MP_DEV int fb_index(int x, int y, int grid_w)
+{
+    return (((x >> 3) + grid_w * (y >> 3)) << 6) + (x & 7) + ((y & 7) << 3);
+}
+
+}  // namespace mp

@@ -1,0 +1,66 @@
+// Kernel launch wrappers of the REYES hot path (implemented in mp_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mp_types.h"
+
+namespace mp {
+
+struct SplitQueues {
+    RangeRec* queue;         // private LIFO queues, stride entries each
+    int32_t* queue_cnt;      // entries per private queue
+    int64_t stride;
+    RangeRec* shared_q;      // BALANCED: one shared LIFO queue
+    int32_t* shared_cnt;
+    int32_t n_groups;        // persistent work-groups
+    int32_t stack_cap;       // LDS stack entries per work-group
+};
+
+struct CanonBuffers {
+    RangeRec* draw;          // DRAW records in emission order
+    RangeRec* bucket;        // grouped by global patch index
+    RangeRec* canon;         // canonical order
+    uint32_t cap;            // capacity of the three lists
+    uint32_t* counts;        // per global patch: DRAW count
+    uint32_t* offsets;       // exclusive scan of counts (n_gpids + 1)
+    uint32_t* cursor;        // scatter cursors
+    uint32_t n_gpids;
+};
+
+hipError_t launch_reset(FrameCounters* fc, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+
+hipError_t launch_init_queues(const DevObject* objs, int n_objs, uint32_t total_roots,
+                              const SplitQueues& q, int balanced, hipStream_t s);
+hipError_t launch_split_persistent(const DevObject* objs, const DevView* views,
+                                   const DevParams& prm, const SplitQueues& q,
+                                   const CanonBuffers& cb, FrameCounters* fc, int balanced,
+                                   hipStream_t s);
+
+// BREADTH / BOUNDED segment engine (one level or batch).
+hipError_t launch_bound_segment(const DevObject* objs, const DevView* views,
+                                const DevParams& prm, const RangeRec* in, RangeRec* pad,
+                                uint32_t n, uint8_t* flags, uint32_t* blk_counts,
+                                FrameCounters* fc, hipStream_t s);
+hipError_t launch_scan_blocks(uint32_t* blk_counts, uint32_t n_blocks, uint32_t* totals,
+                              hipStream_t s);
+hipError_t launch_move_segment(const DevObject* objs, const RangeRec* pad, uint32_t n,
+                               const uint8_t* flags, const uint32_t* blk_offsets,
+                               RangeRec* draw_out, uint32_t draw_base, uint32_t draw_limit,
+                               RangeRec* split_out, uint32_t* counts, hipStream_t s);
+uint32_t segment_blocks(uint32_t n);
+
+hipError_t launch_canonicalize(const DevObject* objs, DevView* views, int n_views,
+                               const CanonBuffers& cb, FrameCounters* fc, int num_cus,
+                               hipStream_t s);
+hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject* objs,
+                         const DevView* views, const DevParams& prm, FrameCounters* fc,
+                         float* quad_store, int grid, hipStream_t s);
+int raster_grid(int patch_size, int num_cus);
+hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
+                          const RangeRec* canon, const DevObject* objs, const float* quad_store,
+                          int patch_size, int write_rgba32, hipStream_t s);
+
+}  // namespace mp

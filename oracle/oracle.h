@@ -1,0 +1,172 @@
+/*
+ * CPU oracle for the micropolis REYES hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * product path is compared against.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it; the product library
+ * (micropolis_amd/libmicropolis_hip.so) never links or calls it.
+ *
+ * It is a scalar, IEEE-strict C restatement of the reference OpenCL kernels
+ * (kernels/utility.h, kernels/bound_n_split*.cl, kernels/dice.cl,
+ * kernels/reyes.cl, kernels/framebuffer.cl, kernels/prefix_sum.cl) and of the
+ * host schedulers (src/Reyes/BoundNSplitCL{Breadth,Bounded,Local}.cpp,
+ * src/Reyes/RendererCL.cpp).  Every floating-point expression keeps the
+ * operation order written in the reference source, built with
+ * -ffp-contract=off and correctly rounded '/' and sqrt.  The semantic
+ * decisions the reference leaves open (fast-math, float->int overflow,
+ * depth ties, NaN in min/max) are fixed as documented next to each function.
+ *
+ * Parity status per function (how each is pinned) is listed in oracle.c's
+ * header comment.
+ */
+#ifndef MP_ORACLE_H
+#define MP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_CULL   0
+#define OR_DRAW   1
+#define OR_HSPLIT 2
+#define OR_VSPLIT 6
+
+#define OR_BEZIER  0
+#define OR_GREGORY 1
+
+/* bound_n_split_method values (src/Reyes/config.xml:34-39 plus BALANCED) */
+#define OR_METHOD_CPU      0
+#define OR_METHOD_BOUNDED  1
+#define OR_METHOD_LOCAL    2
+#define OR_METHOD_BREADTH  3
+#define OR_METHOD_BALANCED 4
+
+/* Empty visibility key (no sample) */
+#define OR_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
+
+typedef struct or_view {
+    int32_t width, height;   /* window size in pixels (VIEWPORT_SIZE_PIXEL)      */
+    float proj[16];          /* perspective matrix, column-major                 */
+    float screen[4];         /* screen matrix (mat2), column-major               */
+    float f[2];              /* Projection::f()                                  */
+    float near_, far_;       /* near plane, cull far plane (Projection::far)     */
+    float eps;               /* (float)((double)near * 0.1), bound_n_split.h:57  */
+    float clear_linear[4];   /* framebuffer clear value (rgb^2.2, 1000)          */
+} or_view;
+
+typedef struct or_params {
+    int32_t patch_size;        /* reyes_patch_size (N), multiple of 8           */
+    int32_t max_split_depth;
+    int32_t bound_sample_rate; /* RES                                           */
+    float   cull_ribbon;
+    int32_t backface_culling;
+    int32_t subpixel_bits;     /* PXLCOORD_SHIFT                                */
+    int32_t patches_per_pass;
+    int32_t displacement;      /* 0 = none, 1 = 3D value-noise displacement     */
+    float   disp_amplitude;
+    float   disp_frequency;
+    uint32_t disp_seed;
+    int32_t disp_octaves;
+} or_params;
+
+typedef struct or_range {
+    int32_t pid;
+    int32_t depth;
+    float umin, vmin, umax, vmax;
+} or_range;
+
+/* ---- patch evaluation (kernels/utility.h:41-126) ---- */
+void or_eval_patch(const float* cps, int type, int pid, float u, float v, float out[4]);
+void or_eval_patch_f64(const double* cps, int type, int pid, double u, double v, double out[4]);
+
+/* ---- value noise + displacement (defined by this build; not in the reference) ---- */
+float or_noise(const or_params* prm, float x, float y, float z);
+void  or_displaced_point(const float* cps, int type, int pid, float u, float v,
+                         const or_params* prm, float out[4]);
+
+/* ---- bound (kernels/bound_n_split.h:10-100) ---- */
+int or_bound(const float* cps, int type, int pid, int depth,
+             float umin, float vmin, float umax, float vmax,
+             const float mv[16], const or_view* view, const or_params* prm,
+             float split_limit);
+
+/* ---- split schedules ----
+ * Returns the number of DRAW ranges written to out (all passes concatenated
+ * in emission order), or -1 if out_cap is too small.  pass_counts[k] is the
+ * number of DRAW ranges emitted by pass k (BREADTH: level k; BOUNDED: batch k;
+ * LOCAL/BALANCED: 1 pass with the full set), *n_passes the pass count,
+ * *n_bounds the number of bound() evaluations.  For BREADTH the ranges the
+ * renderer drops (RendererCL.cpp:209) are NOT written; pass_counts holds the
+ * untruncated counts and *n_dropped the dropped total. */
+long or_split(int method, const float* cps, int type, int n_patches,
+              const float mv[16], const or_view* view, const or_params* prm,
+              float split_limit, or_range* out, long out_cap,
+              int32_t* pass_counts, int max_passes, int32_t* n_passes,
+              long* n_bounds, long* n_dropped);
+
+/* Literal single-threaded emulation of the LOCAL persistent kernel
+ * (kernels/bound_n_split_local.cl) with work-group size wgs and wgc groups,
+ * groups run one after another.  Used to show its DRAW multiset equals the
+ * canonical set.  Same return convention as or_split. */
+long or_split_local_emulated(const float* cps, int type, int n_patches,
+                             const float mv[16], const or_view* view, const or_params* prm,
+                             float split_limit, int wgs, int wgc, long batch_size,
+                             or_range* out, long out_cap, int32_t* pass_counts, int max_passes,
+                             int32_t* n_passes, long* n_bounds);
+
+/* Sort ranges into canonical order: (pid, umin, vmin). */
+void or_canonical_sort(or_range* r, long n);
+uint64_t or_canonical_key(const or_range* r);
+
+/* ---- dice / shade / sample (kernels/dice.cl, kernels/reyes.cl) ---- */
+void or_dice(const float* cps, int type, const or_range* r, const float mv[16],
+             const or_view* view, const or_params* prm,
+             float* pos, int32_t* pxl, float* depth);
+void or_shade(const float* pos, const int32_t* pxl, const or_view* view, const or_params* prm,
+              int32_t* block_index, float* quad_s);
+void or_sample(const int32_t* block_index, const int32_t* pxl, const float* depth,
+               const or_view* view, const or_params* prm, uint32_t id_base,
+               uint64_t* keybuf);
+
+/* dice+shade+sample of n canonical ranges; range i gets quad ids
+ * (range_base+i)*N*N + v*N + u; quad_store receives one float per quad
+ * at the same index (shade's lambert term, NaN when the colour is NaN). */
+void or_render(const float* cps, int type, const or_range* ranges, long n,
+               uint32_t range_base, const float mv[16], const or_view* view,
+               const or_params* prm, uint64_t* keybuf, float* quad_store);
+
+/* Resolve the key buffer.  range_color[r] = diffuse colour of range r.
+ * Outputs (any may be NULL), all linear row-major W*H:
+ *   rgba8  : display colour (gamma 1/2.2 as shaders/tex_draw.frag)
+ *   depth  : FP32 depth of the visible sample, 1.0 where empty
+ *   ids    : visible quad id, 0xFFFFFFFF where empty
+ *   rgba32 : linear framebuffer colour (RGBA32F as the reference buffer) */
+void or_resolve(const uint64_t* keybuf, const float* quad_store, const float* range_color,
+                const or_view* view, const or_params* prm,
+                uint8_t* rgba8, float* depth, uint32_t* ids, float* rgba32);
+
+/* Projection (src/base/Projection.cpp:23-50, glm::perspective with far 10000,
+ * cull far 1000, Projection.h:49) and screen matrix; clear colour stored as
+ * (rgb^2.2, 1000) (src/Reyes/Framebuffer.cpp:65-68). */
+void or_make_view(float fovy_deg, float near_, int width, int height,
+                  const float clear_color[4], or_view* out);
+
+int  or_framebuffer_index(int x, int y, int width);
+long or_framebuffer_size(int width, int height);
+
+/* Display transform of one linear channel to 8 bits. */
+uint8_t or_to_srgb8(float c);
+
+/* Inclusive prefix sum of int32 (kernels/prefix_sum.cl semantics) */
+int32_t or_prefix_sum(const int32_t* in, int32_t* out, long n);
+
+/* glibc-compatible rand() sequence for the reference's scan self-test
+ * (main.cpp:343-400 uses srand(43); rand()%8+1). */
+void or_glibc_rand(uint32_t seed, int32_t* out, long n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
