@@ -1,0 +1,119 @@
+"""Benchmark: plain 1080p REYES frame of the 4096-patch displaced torus.
+
+Workload: 4096 bicubic Bezier patches on a 64x64 torus (R=1, r=0.35),
+3D value-noise displacement (4 octaves, amplitude 0.02), 1920x1080,
+LOCAL bound-and-split, bound_n_split_limit 16, reyes_patch_size 16.
+
+A step is one whole frame with the patches resident in HBM: persistent
+bound-and-split, canonical ordering, fused dice/shade/sample, resolve to
+RGBA8 + FP32 depth + visible ids (the frame's kernels replayed as one hipGraph).
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only; this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from micropolis_amd import Renderer, scenes
+
+    w = scenes.c2_torus()
+    v = w.views[0]
+    r = Renderer(**w.options)
+    r.prepare()
+    for o in v.objects:
+        r.load_patches(o.handle, o.patches, o.patch_type)
+        r.draw_patches(o.handle, o.matrix, v.projection, o.color)
+    r.finish()
+    st = r.stats()
+    use_graph = not args.no_graph
+    print("first frame: %d ranges, %d micropolygons, %d coverage samples" % (
+        st["diced_ranges"], st["micropolygons"], st["coverage_samples"]), file=sys.stderr, flush=True)
+
+    if args.warmup:
+        r.replay(args.warmup, use_graph=use_graph)
+    t0 = time.perf_counter()
+    ms_dev = r.replay(args.steps, use_graph=use_graph)
+    t1 = time.perf_counter()
+    ms_wall = (t1 - t0) * 1e3 / args.steps
+    stages = r.replay_stages(max(3, min(args.steps, 20)))
+    print("frame %.3f ms (device %.3f ms); stages %s" % (ms_wall, ms_dev, stages),
+          file=sys.stderr, flush=True)
+
+    mpolys = st["micropolygons"]
+    value = mpolys / (ms_wall * 1e-3) / 1e6
+
+    # roofline of the dominant kernel (fused dice + shade + sample): algorithmic
+    # bytes = range records (24 B) + patch control points once (16 x 16 B) +
+    # per-micropolygon colour term (4 B) + one 8-byte key atomic per sample
+    N = w.options["reyes_patch_size"]
+    P = w.patch_count
+    alg_bytes = st["diced_ranges"] * (24 + 4 * N * N) + P * 16 * 16 + st["coverage_samples"] * 8
+    raster_ms = stages["raster"]
+    achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle
+        t = time.perf_counter()
+        frames = 0
+        while True:
+            ref = pyoracle.render(w)[0]
+            frames += 1
+            if time.perf_counter() - t > 8.0 or frames >= 3:
+                break
+        dt = (time.perf_counter() - t) / frames
+        n_ranges = len(ref["ranges"])
+        cpu = {"value": round(n_ranges * N * N / dt / 1e6, 4), "unit": "Mmicropolys/s", "cores": 1,
+               "kind": "port",
+               "sample": "%d full frame(s) of the same workload through the scalar C oracle, "
+                         "%.2f s/frame" % (frames, dt)}
+
+    line = {
+        "metric": "1080p frame ms + Mmicropolys/s (plain LOCAL frame, displaced torus)",
+        "value": round(value, 3),
+        "unit": "Mmicropolys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded torus patches, value-noise displacement)",
+        "config": {"workload": w.name, "patches": P, "width": 1920, "height": 1080,
+                   "reyes_patch_size": N, "bound_n_split_limit": 16, "method": "LOCAL",
+                   "graph": use_graph, "ranges": st["diced_ranges"], "micropolygons": mpolys,
+                   "coverage_samples": st["coverage_samples"]},
+        "stages_ms": {k: round(x, 4) for k, x in stages.items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_raster<16>", "alg_bytes": int(alg_bytes)},
+        "cpu_baseline": cpu,
+    }
+    r.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -71,6 +71,8 @@ typedef struct mp_config {
     uint32_t displacement_seed;
     int32_t displacement_octaves;
     int32_t dummy_render;                  /* split only, no raster           */
+    int32_t write_rgba32;                  /* resolve also writes the linear  */
+                                           /* RGBA32F framebuffer             */
 } mp_config;
 
 /* Reyes::Projection constructor arguments (src/base/Projection.h:44) */
@@ -91,6 +93,7 @@ typedef struct mp_stats {
     int64_t max_patches;              /* largest in-flight split level/stack   */
     int64_t slow_blocks;              /* raster blocks on the wide-int path    */
     int64_t overflow_relaunches;      /* persistent split relaunches           */
+    int64_t coverage_samples;         /* (micropolygon, pixel) samples written */
     double  frame_ms;                 /* mp_prepare..mp_finish device time     */
 } mp_stats;
 
@@ -164,6 +167,10 @@ int mp_read_pass_counts(mp_context* ctx, int32_t* counts, int32_t capacity,
 /* Statistics (src/base/Statistics.cpp:160-218) of the last frame. */
 int mp_get_stats(mp_context* ctx, mp_stats* stats);
 
+/* Displacement amplitude used by subsequent draws (overrides
+ * mp_config.displacement_amplitude).  No reference counterpart. */
+int mp_set_displacement_amplitude(mp_context* ctx, float amplitude);
+
 /* ---- batching: many independent views/scenes rendered by one set of
  * persistent launches (no reference counterpart; the plain path is the
  * 1-view case of the same machinery). ---- */
@@ -184,6 +191,11 @@ int mp_batch_read(mp_context* ctx, int32_t view, uint8_t* rgba8, float* depth,
  * on the device (inputs resident), returning the mean device ms per replay
  * measured with HIP events around the replays. ---- */
 int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* ms_per_frame);
+
+/* Replay the last LOCAL/BALANCED frame n times with HIP events between its
+ * stages on the library's stream; stage_ms[0..3] receive the mean device ms
+ * of split, canonical ordering, raster (dice+shade+sample) and resolve. */
+int mp_replay_stages(mp_context* ctx, int32_t n, double stage_ms[4]);
 
 #ifdef __cplusplus
 }

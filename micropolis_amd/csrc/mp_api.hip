@@ -1,0 +1,1034 @@
+// Host side of the C ABI: patch index, frame orchestration, batching, replay.
+//
+// Mirrors Reyes::RendererCL (src/Reyes/RendererCL.cpp), the BoundNSplitCL*
+// schedulers (src/Reyes/BoundNSplitCL{Breadth,Bounded,Local}.cpp),
+// Reyes::PatchIndex (src/Reyes/PatchIndex.cpp), Reyes::Projection
+// (src/base/Projection.cpp) and Reyes::Framebuffer (src/Reyes/Framebuffer.cpp).
+//
+// A frame is recorded by mp_prepare / mp_draw_patches and executed by
+// mp_finish: all objects of all views are split together (one persistent
+// launch for LOCAL/BALANCED, a host-driven level loop for BREADTH/BOUNDED),
+// the DRAW ranges are put in canonical order, rasterised by one persistent
+// fused dice/shade/sample launch and resolved per view.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/micropolis.h"
+#include "mp_kernels.h"
+#include "mp_types.h"
+
+using namespace mp;
+
+namespace {
+
+std::string g_create_error;
+
+struct PatchSet {
+    float4* d_cps = nullptr;
+    int type = 0;
+    int n_patches = 0;
+};
+
+struct ObjectReq {
+    uint64_t handle;
+    int view;
+    float mv[16];
+    float color[4];
+    float amplitude;
+};
+
+struct ViewBuffers {
+    int width = 0, height = 0;
+    int grid_w = 0, grid_h = 0;
+    unsigned long long* keys = nullptr;
+    uint8_t* rgba8 = nullptr;
+    float* depth = nullptr;
+    uint32_t* ids = nullptr;
+    float* rgba32 = nullptr;
+};
+
+}  // namespace
+
+struct mp_context {
+    mp_config cfg;
+    DevParams prm;
+    std::string err;
+    int device = 0;
+    int num_cus = 1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float amplitude = 0.0f;
+
+    std::unordered_map<uint64_t, PatchSet> patches;
+
+    // recorded frame
+    int state = 0;  // 0 idle, 1 frame, 2 batch
+    std::vector<mp_projection> view_projs;
+    std::vector<ObjectReq> objects;
+
+    // per-view device buffers
+    std::vector<ViewBuffers> views;
+
+    // device tables (+ pinned host staging)
+    DevView* d_views = nullptr; DevView* h_views = nullptr; int views_cap = 0;
+    DevObject* d_objs = nullptr; DevObject* h_objs = nullptr; int objs_cap = 0;
+    FrameCounters* d_fc = nullptr; FrameCounters* h_fc = nullptr;
+    uint32_t* d_totals = nullptr; uint32_t* h_totals = nullptr;
+
+    // persistent split queues
+    RangeRec* d_queue = nullptr; size_t queue_entries = 0;
+    int32_t* d_queue_cnt = nullptr; int queue_groups_cap = 0;
+    RangeRec* d_shared_q = nullptr; size_t shared_entries = 0;
+    int32_t* d_shared_cnt = nullptr;
+
+    // segment engine (BREADTH / BOUNDED)
+    RangeRec* d_seg_a = nullptr; size_t seg_a_cap = 0;
+    RangeRec* d_seg_b = nullptr; size_t seg_b_cap = 0;
+    uint8_t* d_flags = nullptr; size_t flags_cap = 0;
+    uint32_t* d_blk = nullptr; size_t blk_cap = 0;
+
+    // canonical lists
+    RangeRec* d_draw = nullptr; RangeRec* d_bucket = nullptr; RangeRec* d_canon = nullptr;
+    uint32_t list_cap = 0;
+    uint32_t* d_counts = nullptr; uint32_t* d_offsets = nullptr; uint32_t* d_cursor = nullptr;
+    uint32_t gpid_cap = 0;
+    float* d_quad = nullptr; size_t quad_cap = 0;
+
+    // last frame
+    uint32_t n_gpids = 0;
+    uint32_t total_roots = 0;
+    int n_views = 0;
+    int max_pixels = 0;
+    int raster_grid_size = 0;
+    int split_groups = 0;
+    int stack_cap = 0;
+    mp_stats stats;
+    std::vector<int32_t> pass_counts;
+    bool frame_valid = false;
+    bool device_driven = false;  // the last frame can be replayed without host syncs
+    hipGraphExec_t graph = nullptr;
+};
+
+#define HIPCHK(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                \
+            return MP_E_HIP;                                                               \
+        }                                                                                  \
+    } while (0)
+
+static int fail(mp_context* ctx, int code, const std::string& msg)
+{
+    ctx->err = msg;
+    return code;
+}
+
+template <typename T>
+static int dev_alloc(mp_context* ctx, T** p, size_t count)
+{
+    if (*p) { hipFree(*p); *p = nullptr; }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return MP_E_NOMEM;
+    }
+    return MP_OK;
+}
+
+// grow a device array keeping its first `keep` elements
+template <typename T>
+static int dev_grow(mp_context* ctx, T** p, size_t* cap, size_t need, size_t keep)
+{
+    if (need <= *cap && *p) return MP_OK;
+    size_t ncap = std::max(need, *cap * 2);
+    T* np = nullptr;
+    hipError_t e = hipMalloc((void**)&np, ncap * sizeof(T));
+    if (e != hipSuccess) return fail(ctx, MP_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (*p && keep) {
+        HIPCHK(ctx, hipMemcpyAsync(np, *p, keep * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (*p) hipFree(*p);
+    *p = np;
+    *cap = ncap;
+    return MP_OK;
+}
+
+static void invalidate_graph(mp_context* ctx)
+{
+    if (ctx->graph) {
+        hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// projection (src/base/Projection.cpp:23-50; glm::perspective, far 10000)
+
+static void make_view(const mp_projection& pr, const float clear[4], int subpixel_bits, DevView& v)
+{
+    memset(&v, 0, sizeof(v));
+    const int W = pr.width, H = pr.height;
+    const float nearp = pr.near_plane;
+    float aspect = (float)W / (float)H;
+    float fovy = (float)((double)pr.fovy_deg * 3.14159265358979323846 / 180.0);
+    float zfar = 10000.0f;
+    float th = tanf(fovy / 2.0f);
+    v.proj[0] = 1.0f / (aspect * th);
+    v.proj[5] = 1.0f / th;
+    v.proj[10] = -(zfar + nearp) / (zfar - nearp);
+    v.proj[11] = -1.0f;
+    v.proj[14] = -(2.0f * zfar * nearp) / (zfar - nearp);
+    float fy = 1.0f / (float)tan((double)pr.fovy_deg * 3.14159265358979323846 / 360.0);
+    float fx = fy / aspect;
+    float vpx = (float)W / 2.0f, vpy = (float)H / 2.0f;
+    fx *= vpx;
+    fy *= vpy;
+    v.f[0] = fx;
+    v.f[1] = fy;
+    v.screen[0] = vpx; v.screen[1] = 0.0f; v.screen[2] = 0.0f; v.screen[3] = vpy;
+    v.near_ = nearp;
+    v.far_ = 1000.0f;
+    v.eps = (float)((double)nearp * 0.1);
+    v.width = W;
+    v.height = H;
+    v.grid_w = (W + 7) / 8;
+    v.grid_h = (H + 7) / 8;
+    v.vs_x = W << subpixel_bits;
+    v.vs_y = H << subpixel_bits;
+    for (int k = 0; k < 3; ++k) v.clear_linear[k] = powf(clear[k], 2.2f);
+    v.clear_linear[3] = 1000.0f;
+}
+
+// ---------------------------------------------------------------------------
+
+static int validate_config(const mp_config* c, std::string& why)
+{
+    if (c->reyes_patch_size != 8 && c->reyes_patch_size != 16 && c->reyes_patch_size != 32) {
+        why = "reyes_patch_size must be 8, 16 or 32";
+        return 0;
+    }
+    if (c->bound_sample_rate != 3) { why = "bound_sample_rate must be 3"; return 0; }
+    if (c->max_split_depth < 1 || c->max_split_depth > 23) { why = "max_split_depth must be in [1,23]"; return 0; }
+    if (c->framebuffer_tile_size != 8) { why = "framebuffer_tile_size must be 8"; return 0; }
+    if (c->subpixel_bits < 0 || c->subpixel_bits > 8) { why = "subpixel_bits must be in [0,8]"; return 0; }
+    if (c->reyes_patches_per_pass < 1) { why = "reyes_patches_per_pass must be positive"; return 0; }
+    if (c->bound_n_split_method < MP_BNS_CPU || c->bound_n_split_method > MP_BNS_BALANCED) {
+        why = "unknown bound_n_split_method";
+        return 0;
+    }
+    if (c->bound_n_split_method == MP_BNS_CPU) {
+        why = "bound_n_split_method CPU is not available in this library";
+        return 0;
+    }
+    if (c->window_width < 1 || c->window_height < 1) { why = "window_size must be positive"; return 0; }
+    if (c->displacement && c->displacement_octaves < 1) { why = "displacement_octaves must be >= 1"; return 0; }
+    return 1;
+}
+
+extern "C" int mp_config_defaults(mp_config* c)
+{
+    if (!c) return MP_E_INVALID;
+    memset(c, 0, sizeof(*c));
+    // reyes.options
+    c->window_width = 1600;
+    c->window_height = 1200;
+    c->clear_color[0] = 0.1f; c->clear_color[1] = 0.1f; c->clear_color[2] = 0.7f; c->clear_color[3] = 0.0f;
+    c->reyes_patch_size = 8;
+    c->bound_n_split_limit = 8.0f;
+    c->cull_ribbon = 32.0f;
+    c->backface_culling = 1;
+    c->bound_n_split_method = MP_BNS_BOUNDED;
+    c->bound_sample_rate = 3;
+    c->max_split_depth = 23;
+    c->reyes_patches_per_pass = 100000;
+    c->framebuffer_tile_size = 8;
+    c->subpixel_bits = 6;
+    c->dice_group_width = 8;
+    c->bns_pipeline_length = 8;
+    c->local_bns_work_groups = 128;
+    c->displacement = 0;
+    c->displacement_amplitude = 0.02f;
+    c->displacement_frequency = 4.0f;
+    c->displacement_seed = 1;
+    c->displacement_octaves = 4;
+    c->dummy_render = 0;
+    c->write_rgba32 = 0;
+    return MP_OK;
+}
+
+extern "C" const char* mp_create_error(void) { return g_create_error.c_str(); }
+
+extern "C" mp_context* mp_create(const mp_config* cfg)
+{
+    if (!cfg) { g_create_error = "null config"; return nullptr; }
+    std::string why;
+    if (!validate_config(cfg, why)) { g_create_error = why; return nullptr; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_error = "no HIP device";
+        return nullptr;
+    }
+    mp_context* ctx = new mp_context();
+    ctx->cfg = *cfg;
+    ctx->amplitude = cfg->displacement_amplitude;
+    DevParams& p = ctx->prm;
+    p.patch_size = cfg->reyes_patch_size;
+    p.max_split_depth = cfg->max_split_depth;
+    p.cull_ribbon = cfg->cull_ribbon;
+    p.backface_culling = cfg->backface_culling;
+    p.subpixel_bits = cfg->subpixel_bits;
+    p.patches_per_pass = cfg->reyes_patches_per_pass;
+    p.displacement = cfg->displacement;
+    p.disp_frequency = cfg->displacement_frequency;
+    p.disp_seed = cfg->displacement_seed;
+    p.disp_octaves = cfg->displacement_octaves;
+
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&ctx->device) != hipSuccess ||
+        hipGetDeviceProperties(&prop, ctx->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_fc, sizeof(FrameCounters)) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_fc, sizeof(FrameCounters), 0) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_totals, 4 * sizeof(uint32_t)) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_totals, 4 * sizeof(uint32_t), 0) != hipSuccess) {
+        g_create_error = "HIP initialisation failed";
+        mp_destroy(ctx);
+        return nullptr;
+    }
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->raster_grid_size = raster_grid(cfg->reyes_patch_size, ctx->num_cus);
+    ctx->stack_cap = 64 * (cfg->max_split_depth + 2);
+    memset(&ctx->stats, 0, sizeof(ctx->stats));
+    return ctx;
+}
+
+static void free_view(ViewBuffers& vb)
+{
+    hipFree(vb.keys); hipFree(vb.rgba8); hipFree(vb.depth); hipFree(vb.ids); hipFree(vb.rgba32);
+    vb = ViewBuffers();
+}
+
+extern "C" void mp_destroy(mp_context* ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    invalidate_graph(ctx);
+    for (auto& kv : ctx->patches) hipFree(kv.second.d_cps);
+    for (auto& vb : ctx->views) free_view(vb);
+    hipFree(ctx->d_views); hipHostFree(ctx->h_views);
+    hipFree(ctx->d_objs); hipHostFree(ctx->h_objs);
+    hipFree(ctx->d_fc); hipHostFree(ctx->h_fc);
+    hipFree(ctx->d_totals); hipHostFree(ctx->h_totals);
+    hipFree(ctx->d_queue); hipFree(ctx->d_queue_cnt); hipFree(ctx->d_shared_q); hipFree(ctx->d_shared_cnt);
+    hipFree(ctx->d_seg_a); hipFree(ctx->d_seg_b); hipFree(ctx->d_flags); hipFree(ctx->d_blk);
+    hipFree(ctx->d_draw); hipFree(ctx->d_bucket); hipFree(ctx->d_canon);
+    hipFree(ctx->d_counts); hipFree(ctx->d_offsets); hipFree(ctx->d_cursor);
+    hipFree(ctx->d_quad);
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char* mp_last_error(const mp_context* ctx)
+{
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int mp_are_patches_loaded(mp_context* ctx, uint64_t handle)
+{
+    if (!ctx) return 0;
+    return ctx->patches.count(handle) ? 1 : 0;
+}
+
+extern "C" int mp_load_patches(mp_context* ctx, uint64_t handle, const float* xyz, size_t n_points,
+                               int32_t patch_type)
+{
+    if (!ctx) return MP_E_INVALID;
+    if (patch_type != MP_BEZIER && patch_type != MP_GREGORY)
+        return fail(ctx, MP_E_INVALID, "unknown patch type");
+    const size_t per = patch_type == MP_GREGORY ? 20 : 16;
+    if (n_points % per != 0) return fail(ctx, MP_E_INVALID, "point count is not a multiple of the patch size");
+    if (n_points && !xyz) return fail(ctx, MP_E_INVALID, "null patch data");
+    if (n_points / per >= (1u << 20))
+        return fail(ctx, MP_E_INVALID, "at most 2^20 - 1 patches per handle");
+    if (patch_type == MP_GREGORY && ctx->cfg.displacement)
+        return fail(ctx, MP_E_INVALID, "displacement is defined for Bezier patches only");
+    // PatchIndex::load_patches: vec3 -> vec4(p, 1)
+    std::vector<float4> host(n_points);
+    for (size_t i = 0; i < n_points; ++i)
+        host[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 1.0f);
+    PatchSet ps;
+    ps.type = patch_type;
+    ps.n_patches = (int)(n_points / per);
+    if (n_points) {
+        HIPCHK(ctx, hipMalloc((void**)&ps.d_cps, n_points * sizeof(float4)));
+        HIPCHK(ctx, hipMemcpy(ps.d_cps, host.data(), n_points * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    auto it = ctx->patches.find(handle);
+    if (it != ctx->patches.end()) hipFree(it->second.d_cps);
+    ctx->patches[handle] = ps;
+    invalidate_graph(ctx);
+    ctx->frame_valid = false;
+    return MP_OK;
+}
+
+extern "C" int mp_unload_patches(mp_context* ctx, uint64_t handle)
+{
+    if (!ctx) return MP_E_INVALID;
+    auto it = ctx->patches.find(handle);
+    if (it == ctx->patches.end()) return fail(ctx, MP_E_NOTFOUND, "unknown patch handle");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(it->second.d_cps);
+    ctx->patches.erase(it);
+    invalidate_graph(ctx);
+    ctx->frame_valid = false;
+    return MP_OK;
+}
+
+extern "C" int mp_set_displacement_amplitude(mp_context* ctx, float amplitude)
+{
+    if (!ctx) return MP_E_INVALID;
+    ctx->amplitude = amplitude;
+    return MP_OK;
+}
+
+extern "C" int mp_prepare(mp_context* ctx)
+{
+    if (!ctx) return MP_E_INVALID;
+    ctx->state = 1;
+    ctx->objects.clear();
+    ctx->view_projs.assign(1, mp_projection{ 60.0f, 0.01f, ctx->cfg.window_width, ctx->cfg.window_height });
+    return MP_OK;
+}
+
+static int add_object(mp_context* ctx, int view, uint64_t handle, const float mv[16], const float color[4])
+{
+    if (!mv || !color) return fail(ctx, MP_E_INVALID, "null matrix or colour");
+    if (!ctx->patches.count(handle)) return fail(ctx, MP_E_NOTFOUND, "unknown patch handle");
+    ObjectReq o;
+    o.handle = handle;
+    o.view = view;
+    memcpy(o.mv, mv, sizeof(o.mv));
+    memcpy(o.color, color, sizeof(o.color));
+    o.amplitude = ctx->amplitude;
+    ctx->objects.push_back(o);
+    return MP_OK;
+}
+
+extern "C" int mp_draw_patches(mp_context* ctx, uint64_t handle, const float mv[16],
+                               const mp_projection* projection, const float color[4])
+{
+    if (!ctx) return MP_E_INVALID;
+    if (ctx->state != 1) return fail(ctx, MP_E_STATE, "mp_draw_patches outside mp_prepare/mp_finish");
+    if (!projection) return fail(ctx, MP_E_INVALID, "null projection");
+    if (projection->width != ctx->cfg.window_width || projection->height != ctx->cfg.window_height)
+        return fail(ctx, MP_E_INVALID, "projection viewport must equal the window size");
+    if (ctx->objects.empty()) ctx->view_projs[0] = *projection;
+    return add_object(ctx, 0, handle, mv, color);
+}
+
+// ---------------------------------------------------------------------------
+// frame execution
+
+static int ensure_views(mp_context* ctx)
+{
+    const int nv = (int)ctx->view_projs.size();
+    if ((int)ctx->views.size() < nv) ctx->views.resize(nv);
+    ctx->max_pixels = 0;
+    for (int v = 0; v < nv; ++v) {
+        const mp_projection& pr = ctx->view_projs[v];
+        ViewBuffers& vb = ctx->views[v];
+        const bool want32 = ctx->cfg.write_rgba32 != 0;
+        if (vb.width != pr.width || vb.height != pr.height || (want32 && !vb.rgba32)) {
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            free_view(vb);
+            vb.width = pr.width;
+            vb.height = pr.height;
+            vb.grid_w = (pr.width + 7) / 8;
+            vb.grid_h = (pr.height + 7) / 8;
+            const size_t nk = (size_t)vb.grid_w * vb.grid_h * 64;
+            const size_t np = (size_t)pr.width * pr.height;
+            int rc;
+            if ((rc = dev_alloc(ctx, &vb.keys, nk)) || (rc = dev_alloc(ctx, &vb.rgba8, np * 4)) ||
+                (rc = dev_alloc(ctx, &vb.depth, np)) || (rc = dev_alloc(ctx, &vb.ids, np)))
+                return rc;
+            if (want32 && (rc = dev_alloc(ctx, &vb.rgba32, np * 4))) return rc;
+            HIPCHK(ctx, launch_fill_u64(vb.keys, nk, MP_EMPTY_KEY, ctx->stream));
+            invalidate_graph(ctx);
+        }
+        ctx->max_pixels = std::max(ctx->max_pixels, vb.grid_w * vb.grid_h * 64);
+    }
+    return MP_OK;
+}
+
+static int ensure_lists(mp_context* ctx, uint32_t cap)
+{
+    if (cap <= ctx->list_cap && ctx->d_draw) return MP_OK;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_draw, cap)) || (rc = dev_alloc(ctx, &ctx->d_bucket, cap)) ||
+        (rc = dev_alloc(ctx, &ctx->d_canon, cap)))
+        return rc;
+    const size_t q = (size_t)cap * ctx->cfg.reyes_patch_size * ctx->cfg.reyes_patch_size;
+    if ((rc = dev_alloc(ctx, &ctx->d_quad, q))) return rc;
+    ctx->quad_cap = q;
+    ctx->list_cap = cap;
+    invalidate_graph(ctx);
+    return MP_OK;
+}
+
+// grow the range lists to at least `need`, keeping the first `keep` DRAW
+// records (the segment engine appends level by level)
+static int grow_lists(mp_context* ctx, uint32_t need, uint32_t keep)
+{
+    if (need <= ctx->list_cap && ctx->d_draw) return MP_OK;
+    size_t cap = ctx->list_cap;
+    int rc;
+    if ((rc = dev_grow(ctx, &ctx->d_draw, &cap, need, keep))) return rc;
+    const uint32_t ncap = (uint32_t)cap;
+    if ((rc = dev_alloc(ctx, &ctx->d_bucket, ncap)) || (rc = dev_alloc(ctx, &ctx->d_canon, ncap))) return rc;
+    const size_t q = (size_t)ncap * ctx->cfg.reyes_patch_size * ctx->cfg.reyes_patch_size;
+    if ((rc = dev_alloc(ctx, &ctx->d_quad, q))) return rc;
+    ctx->quad_cap = q;
+    ctx->list_cap = ncap;
+    invalidate_graph(ctx);
+    return MP_OK;
+}
+
+static int ensure_gpids(mp_context* ctx, uint32_t n)
+{
+    if (n + 1 <= ctx->gpid_cap && ctx->d_counts) return MP_OK;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t cap = std::max<uint32_t>(n + 1, 1024);
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_counts, cap)) || (rc = dev_alloc(ctx, &ctx->d_offsets, cap)) ||
+        (rc = dev_alloc(ctx, &ctx->d_cursor, cap)))
+        return rc;
+    HIPCHK(ctx, launch_fill_u32(ctx->d_counts, cap, 0, ctx->stream));
+    HIPCHK(ctx, launch_fill_u32(ctx->d_cursor, cap, 0, ctx->stream));
+    ctx->gpid_cap = cap;
+    invalidate_graph(ctx);
+    return MP_OK;
+}
+
+static CanonBuffers canon_buffers(mp_context* ctx)
+{
+    CanonBuffers cb;
+    cb.draw = ctx->d_draw;
+    cb.bucket = ctx->d_bucket;
+    cb.canon = ctx->d_canon;
+    cb.cap = ctx->list_cap;
+    cb.counts = ctx->d_counts;
+    cb.offsets = ctx->d_offsets;
+    cb.cursor = ctx->d_cursor;
+    cb.n_gpids = ctx->n_gpids;
+    return cb;
+}
+
+static int upload_tables(mp_context* ctx)
+{
+    const int nv = (int)ctx->view_projs.size();
+    const int no = (int)ctx->objects.size();
+    if (nv > ctx->views_cap) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        hipFree(ctx->d_views); hipHostFree(ctx->h_views);
+        ctx->d_views = nullptr; ctx->h_views = nullptr;
+        HIPCHK(ctx, hipMalloc((void**)&ctx->d_views, nv * sizeof(DevView)));
+        HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_views, nv * sizeof(DevView), 0));
+        ctx->views_cap = nv;
+        invalidate_graph(ctx);
+    }
+    if (std::max(no, 1) > ctx->objs_cap) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        hipFree(ctx->d_objs); hipHostFree(ctx->h_objs);
+        ctx->d_objs = nullptr; ctx->h_objs = nullptr;
+        int cap = std::max(no, 1);
+        HIPCHK(ctx, hipMalloc((void**)&ctx->d_objs, cap * sizeof(DevObject)));
+        HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_objs, cap * sizeof(DevObject), 0));
+        ctx->objs_cap = cap;
+        invalidate_graph(ctx);
+    }
+    // the previous frame may still read the staging buffers
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    // objects are ordered by view so that each view owns a contiguous
+    // range of global patch indices (stable within a view = draw order)
+    std::vector<int> order(no);
+    for (int i = 0; i < no; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return ctx->objects[a].view < ctx->objects[b].view; });
+    std::vector<uint32_t> first(nv, 0), end(nv, 0);
+    uint32_t g = 0;
+    int oi = 0;
+    for (int v = 0; v < nv; ++v) {
+        first[v] = g;
+        while (oi < no && ctx->objects[order[oi]].view == v) {
+            const ObjectReq& o = ctx->objects[order[oi]];
+            const PatchSet& ps = ctx->patches[o.handle];
+            DevObject& d = ctx->h_objs[oi];
+            d.cps = ps.d_cps;
+            memcpy(d.mv, o.mv, sizeof(d.mv));
+            memcpy(d.color, o.color, sizeof(d.color));
+            d.split_limit = ctx->cfg.bound_n_split_limit;
+            d.disp_amplitude = o.amplitude;
+            d.type = ps.type;
+            d.n_patches = ps.n_patches;
+            d.view = v;
+            d.gpid_base = g;
+            g += (uint32_t)ps.n_patches;
+            ++oi;
+        }
+        end[v] = g;
+    }
+    ctx->n_gpids = g;
+    ctx->total_roots = g;
+    for (int v = 0; v < nv; ++v) {
+        DevView& dv = ctx->h_views[v];
+        make_view(ctx->view_projs[v], ctx->cfg.clear_color, ctx->cfg.subpixel_bits, dv);
+        const ViewBuffers& vb = ctx->views[v];
+        dv.keys = vb.keys;
+        dv.rgba8 = vb.rgba8;
+        dv.depth = vb.depth;
+        dv.ids = vb.ids;
+        dv.rgba32 = vb.rgba32;
+        dv.first_gpid = first[v];
+        dv.gpid_end = end[v];
+    }
+    if (nv) HIPCHK(ctx, hipMemcpyAsync(ctx->d_views, ctx->h_views, nv * sizeof(DevView), hipMemcpyHostToDevice, ctx->stream));
+    if (no) HIPCHK(ctx, hipMemcpyAsync(ctx->d_objs, ctx->h_objs, no * sizeof(DevObject), hipMemcpyHostToDevice, ctx->stream));
+    ctx->n_views = nv;
+    return MP_OK;
+}
+
+static int ensure_queues(mp_context* ctx)
+{
+    const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
+    int groups = ctx->cfg.local_bns_work_groups;
+    if (groups <= 0) {
+        groups = ctx->num_cus * split_persistent_occupancy((size_t)ctx->stack_cap * sizeof(RangeRec));
+    }
+    ctx->split_groups = groups;
+    const size_t stride = balanced ? 1 : (ctx->total_roots + groups - 1) / groups;
+    const size_t entries = std::max<size_t>(1, stride * groups);
+    int rc;
+    if (entries > ctx->queue_entries) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = dev_alloc(ctx, &ctx->d_queue, entries))) return rc;
+        ctx->queue_entries = entries;
+        invalidate_graph(ctx);
+    }
+    if (groups > ctx->queue_groups_cap) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = dev_alloc(ctx, &ctx->d_queue_cnt, groups))) return rc;
+        ctx->queue_groups_cap = groups;
+        invalidate_graph(ctx);
+    }
+    if (!ctx->d_shared_cnt && (rc = dev_alloc(ctx, &ctx->d_shared_cnt, 1))) return rc;
+    if (balanced && ctx->total_roots > ctx->shared_entries) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = dev_alloc(ctx, &ctx->d_shared_q, ctx->total_roots))) return rc;
+        ctx->shared_entries = ctx->total_roots;
+        invalidate_graph(ctx);
+    }
+    return MP_OK;
+}
+
+static SplitQueues split_queues(mp_context* ctx)
+{
+    SplitQueues q;
+    const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
+    q.queue = ctx->d_queue;
+    q.queue_cnt = ctx->d_queue_cnt;
+    q.stride = balanced ? 0 : (int64_t)((ctx->total_roots + ctx->split_groups - 1) / ctx->split_groups);
+    q.shared_q = ctx->d_shared_q;
+    q.shared_cnt = ctx->d_shared_cnt;
+    q.n_groups = ctx->split_groups;
+    q.stack_cap = ctx->stack_cap;
+    return q;
+}
+
+// kernels that follow the split: canonical order, raster, resolve; an event
+// is recorded after each stage when `ev` is given
+static int launch_back_end(mp_context* ctx, hipEvent_t* ev = nullptr)
+{
+    CanonBuffers cb = canon_buffers(ctx);
+    HIPCHK(ctx, launch_canonicalize(ctx->d_objs, ctx->d_views, ctx->n_views, cb, ctx->d_fc,
+                                    ctx->num_cus, ctx->stream));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
+    if (!ctx->cfg.dummy_render)
+        HIPCHK(ctx, launch_raster(ctx->cfg.reyes_patch_size, ctx->d_canon, ctx->d_objs, ctx->d_views,
+                                  ctx->prm, ctx->d_fc, ctx->d_quad, ctx->raster_grid_size, ctx->stream));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[1], ctx->stream));
+    if (!ctx->cfg.dummy_render)
+        HIPCHK(ctx, launch_resolve(ctx->d_views, ctx->n_views, ctx->max_pixels, ctx->d_canon, ctx->d_objs,
+                                   ctx->d_quad, ctx->cfg.reyes_patch_size, ctx->cfg.write_rgba32,
+                                   ctx->stream));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[2], ctx->stream));
+    return MP_OK;
+}
+
+// the device-driven frame (LOCAL / BALANCED): no host synchronisation
+static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
+{
+    const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
+    HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->stream));
+    SplitQueues q = split_queues(ctx);
+    if (ctx->total_roots) {
+        HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
+                                       balanced, ctx->stream));
+        HIPCHK(ctx, launch_split_persistent(ctx->d_objs, ctx->d_views, ctx->prm, q,
+                                            canon_buffers(ctx), ctx->d_fc, balanced, ctx->stream));
+    }
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[1], ctx->stream));
+    return launch_back_end(ctx, ev ? ev + 2 : nullptr);
+}
+
+// BREADTH / BOUNDED: host-driven loop per object (one sync per level/batch)
+static int run_segment_split(mp_context* ctx, uint32_t& draw_total, int64_t& dropped, int64_t& max_patches)
+{
+    const bool breadth = ctx->cfg.bound_n_split_method == MP_BNS_BREADTH;
+    const uint32_t ppp = (uint32_t)ctx->cfg.reyes_patches_per_pass;
+    ctx->pass_counts.clear();
+    int rc;
+    for (int oi = 0; oi < (int)ctx->objects.size(); ++oi) {
+        const DevObject& ob = ctx->h_objs[oi];
+        const uint32_t P = (uint32_t)ob.n_patches;
+        if (P == 0) continue;
+        if (breadth) {
+            if ((rc = dev_grow(ctx, &ctx->d_seg_a, &ctx->seg_a_cap, P, 0))) return rc;
+            HIPCHK(ctx, launch_init_roots(oi, P, ctx->d_seg_a, ctx->stream));
+            uint32_t n = P;
+            while (n > 0) {
+                max_patches = std::max<int64_t>(max_patches, n);
+                const uint32_t nb = segment_blocks(n);
+                if ((rc = dev_grow(ctx, &ctx->d_seg_b, &ctx->seg_b_cap, (size_t)2 * n, 0)) ||
+                    (rc = dev_grow(ctx, &ctx->d_flags, &ctx->flags_cap, n, 0)) ||
+                    (rc = dev_grow(ctx, &ctx->d_blk, &ctx->blk_cap, (size_t)2 * nb, 0)))
+                    return rc;
+                if ((rc = grow_lists(ctx, draw_total + std::min(n, ppp), draw_total))) return rc;
+                HIPCHK(ctx, launch_bound_segment(ctx->d_objs, ctx->d_views, ctx->prm, ctx->d_seg_a,
+                                                 ctx->d_seg_a, n, ctx->d_flags, ctx->d_blk, ctx->d_fc,
+                                                 ctx->stream));
+                HIPCHK(ctx, launch_scan_blocks(ctx->d_blk, nb, ctx->d_totals, ctx->stream));
+                HIPCHK(ctx, launch_move_segment(ctx->d_objs, ctx->d_seg_a, n, ctx->d_flags, ctx->d_blk,
+                                                ctx->d_draw, draw_total, ppp, ctx->d_seg_b,
+                                                ctx->d_counts, ctx->stream));
+                HIPCHK(ctx, hipMemcpyAsync(ctx->h_totals, ctx->d_totals, 2 * sizeof(uint32_t),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+                HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+                const uint32_t draws = ctx->h_totals[0], splits = ctx->h_totals[1];
+                ctx->pass_counts.push_back((int32_t)draws);
+                const uint32_t emitted = std::min(draws, ppp);
+                dropped += draws - emitted;
+                draw_total += emitted;
+                n = 2 * splits;
+                std::swap(ctx->d_seg_a, ctx->d_seg_b);
+                std::swap(ctx->seg_a_cap, ctx->seg_b_cap);
+            }
+        } else {
+            // BOUNDED: LIFO stack, batches of reyes_patches_per_pass
+            const size_t scap = (size_t)P + (size_t)2 * ppp;
+            if ((rc = dev_grow(ctx, &ctx->d_seg_a, &ctx->seg_a_cap, scap, 0)) ||
+                (rc = dev_grow(ctx, &ctx->d_seg_b, &ctx->seg_b_cap, ppp, 0)))
+                return rc;
+            HIPCHK(ctx, launch_init_roots(oi, P, ctx->d_seg_a, ctx->stream));
+            uint32_t height = P;
+            while (height > 0) {
+                max_patches = std::max<int64_t>(max_patches, height);
+                const uint32_t b = std::min(height, ppp);
+                height -= b;
+                const uint32_t nb = segment_blocks(b);
+                if ((rc = dev_grow(ctx, &ctx->d_seg_a, &ctx->seg_a_cap, (size_t)height + 2 * b, height + b)) ||
+                    (rc = dev_grow(ctx, &ctx->d_flags, &ctx->flags_cap, b, 0)) ||
+                    (rc = dev_grow(ctx, &ctx->d_blk, &ctx->blk_cap, (size_t)2 * nb, 0)))
+                    return rc;
+                if ((rc = grow_lists(ctx, draw_total + b, draw_total))) return rc;
+                HIPCHK(ctx, launch_bound_segment(ctx->d_objs, ctx->d_views, ctx->prm, ctx->d_seg_a + height,
+                                                 ctx->d_seg_b, b, ctx->d_flags, ctx->d_blk, ctx->d_fc,
+                                                 ctx->stream));
+                HIPCHK(ctx, launch_scan_blocks(ctx->d_blk, nb, ctx->d_totals, ctx->stream));
+                HIPCHK(ctx, launch_move_segment(ctx->d_objs, ctx->d_seg_b, b, ctx->d_flags, ctx->d_blk,
+                                                ctx->d_draw, draw_total, 0xFFFFFFFFu,
+                                                ctx->d_seg_a + height, ctx->d_counts, ctx->stream));
+                HIPCHK(ctx, hipMemcpyAsync(ctx->h_totals, ctx->d_totals, 2 * sizeof(uint32_t),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+                HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+                const uint32_t draws = ctx->h_totals[0], splits = ctx->h_totals[1];
+                ctx->pass_counts.push_back((int32_t)draws);
+                draw_total += draws;
+                height += 2 * splits;
+            }
+        }
+    }
+    return MP_OK;
+}
+
+static int execute_frame(mp_context* ctx)
+{
+    int rc;
+    ctx->frame_valid = false;
+    if ((rc = ensure_views(ctx))) return rc;
+    if ((rc = upload_tables(ctx))) return rc;
+    if ((rc = ensure_gpids(ctx, ctx->n_gpids))) return rc;
+    if (ctx->list_cap == 0 && (rc = ensure_lists(ctx, std::max<uint32_t>(65536, 4 * ctx->total_roots)))) return rc;
+    const int method = ctx->cfg.bound_n_split_method;
+    const bool device_driven = (method == MP_BNS_LOCAL || method == MP_BNS_BALANCED);
+    int64_t relaunches = 0, dropped = 0, max_patches = 0;
+    uint32_t draw_total = 0;
+
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (device_driven) {
+        if ((rc = ensure_queues(ctx))) return rc;
+        for (;;) {
+            if ((rc = launch_device_frame(ctx))) return rc;
+            HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            if (ctx->h_fc->overflow & 2u) return fail(ctx, MP_E_INVALID, "split stack overflow");
+            if (!(ctx->h_fc->overflow & 1u)) break;
+            // draw list too small: grow and render the frame again
+            ++relaunches;
+            if ((rc = ensure_lists(ctx, ctx->list_cap * 2))) return rc;
+            HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        }
+        draw_total = ctx->h_fc->total_ranges;
+        const int64_t ppp = ctx->cfg.reyes_patches_per_pass;
+        ctx->pass_counts.clear();
+        int64_t left = draw_total;
+        do {
+            ctx->pass_counts.push_back((int32_t)std::min<int64_t>(left, ppp));
+            left -= ppp;
+        } while (left > 0);
+        max_patches = ctx->h_fc->max_stack;
+    } else {
+        HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->stream));
+        if ((rc = run_segment_split(ctx, draw_total, dropped, max_patches))) return rc;
+        if ((rc = launch_back_end(ctx))) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+
+    mp_stats& st = ctx->stats;
+    memset(&st, 0, sizeof(st));
+    const int64_t N2 = (int64_t)ctx->cfg.reyes_patch_size * ctx->cfg.reyes_patch_size;
+    st.diced_ranges = ctx->h_fc->total_ranges;
+    st.dropped_ranges = dropped;
+    st.patches_per_frame = st.diced_ranges + dropped;
+    st.bounded_patches = (int64_t)ctx->h_fc->bounds;
+    st.micropolygons = ctx->cfg.dummy_render ? 0 : N2 * st.diced_ranges;
+    st.pass_count = (int64_t)ctx->pass_counts.size();
+    st.max_patches = max_patches;
+    st.slow_blocks = ctx->h_fc->slow_blocks;
+    st.overflow_relaunches = relaunches;
+    st.coverage_samples = (int64_t)ctx->h_fc->atomics;
+    st.frame_ms = ms;
+    ctx->device_driven = device_driven;
+    ctx->frame_valid = true;
+    return MP_OK;
+}
+
+extern "C" int mp_finish(mp_context* ctx)
+{
+    if (!ctx) return MP_E_INVALID;
+    if (ctx->state != 1) return fail(ctx, MP_E_STATE, "mp_finish without mp_prepare");
+    ctx->state = 0;
+    return execute_frame(ctx);
+}
+
+extern "C" int mp_batch_begin(mp_context* ctx, int32_t n_views, const mp_projection* projections)
+{
+    if (!ctx) return MP_E_INVALID;
+    if (n_views < 1 || !projections) return fail(ctx, MP_E_INVALID, "need at least one view");
+    for (int v = 0; v < n_views; ++v)
+        if (projections[v].width < 1 || projections[v].height < 1)
+            return fail(ctx, MP_E_INVALID, "view size must be positive");
+    ctx->state = 2;
+    ctx->objects.clear();
+    ctx->view_projs.assign(projections, projections + n_views);
+    return MP_OK;
+}
+
+extern "C" int mp_batch_draw(mp_context* ctx, int32_t view, uint64_t handle, const float mv[16],
+                             const float color[4])
+{
+    if (!ctx) return MP_E_INVALID;
+    if (ctx->state != 2) return fail(ctx, MP_E_STATE, "mp_batch_draw outside a batch");
+    if (view < 0 || view >= (int)ctx->view_projs.size()) return fail(ctx, MP_E_INVALID, "bad view index");
+    return add_object(ctx, view, handle, mv, color);
+}
+
+extern "C" int mp_batch_finish(mp_context* ctx)
+{
+    if (!ctx) return MP_E_INVALID;
+    if (ctx->state != 2) return fail(ctx, MP_E_STATE, "mp_batch_finish without mp_batch_begin");
+    ctx->state = 0;
+    return execute_frame(ctx);
+}
+
+static int read_view(mp_context* ctx, int view, uint8_t* rgba8, float* depth, uint32_t* ids, float* rgba32)
+{
+    if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame");
+    if (view < 0 || view >= ctx->n_views) return fail(ctx, MP_E_INVALID, "bad view index");
+    if (ctx->cfg.dummy_render) return fail(ctx, MP_E_STATE, "dummy_render frames have no framebuffer");
+    const ViewBuffers& vb = ctx->views[view];
+    const size_t np = (size_t)vb.width * vb.height;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (rgba8) HIPCHK(ctx, hipMemcpy(rgba8, vb.rgba8, np * 4, hipMemcpyDeviceToHost));
+    if (depth) HIPCHK(ctx, hipMemcpy(depth, vb.depth, np * sizeof(float), hipMemcpyDeviceToHost));
+    if (ids) HIPCHK(ctx, hipMemcpy(ids, vb.ids, np * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (rgba32) {
+        if (!vb.rgba32) return fail(ctx, MP_E_STATE, "write_rgba32 is off");
+        HIPCHK(ctx, hipMemcpy(rgba32, vb.rgba32, np * 4 * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return MP_OK;
+}
+
+extern "C" int mp_read_framebuffer(mp_context* ctx, uint8_t* rgba8, float* depth, uint32_t* ids,
+                                   float* rgba32)
+{
+    if (!ctx) return MP_E_INVALID;
+    return read_view(ctx, 0, rgba8, depth, ids, rgba32);
+}
+
+extern "C" int mp_batch_read(mp_context* ctx, int32_t view, uint8_t* rgba8, float* depth,
+                             uint32_t* ids, float* rgba32)
+{
+    if (!ctx) return MP_E_INVALID;
+    return read_view(ctx, view, rgba8, depth, ids, rgba32);
+}
+
+extern "C" int mp_read_ranges(mp_context* ctx, mp_range_record* records, int64_t capacity,
+                              int64_t* count)
+{
+    if (!ctx) return MP_E_INVALID;
+    if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame");
+    const uint32_t n = ctx->h_fc->total_ranges;
+    if (count) *count = n;
+    if (!records) return MP_OK;
+    if (capacity < (int64_t)n) return fail(ctx, MP_E_INVALID, "capacity too small");
+    std::vector<RangeRec> host(n);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (n) HIPCHK(ctx, hipMemcpy(host.data(), ctx->d_canon, n * sizeof(RangeRec), hipMemcpyDeviceToHost));
+    // object slots are view-ordered on the device; report draw-call order
+    std::vector<int> order(ctx->objects.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return ctx->objects[a].view < ctx->objects[b].view; });
+    for (uint32_t i = 0; i < n; ++i) {
+        records[i].pid = (int32_t)host[i].pid;
+        records[i].object = order[host[i].obj_depth & 0xffffffu];
+        records[i].depth = (int32_t)(host[i].obj_depth >> 24);
+        records[i].umin = host[i].umin;
+        records[i].vmin = host[i].vmin;
+        records[i].umax = host[i].umax;
+        records[i].vmax = host[i].vmax;
+    }
+    return MP_OK;
+}
+
+extern "C" int mp_read_pass_counts(mp_context* ctx, int32_t* counts, int32_t capacity, int32_t* n_passes)
+{
+    if (!ctx) return MP_E_INVALID;
+    if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame");
+    const int32_t n = (int32_t)ctx->pass_counts.size();
+    if (n_passes) *n_passes = n;
+    if (counts) {
+        if (capacity < n) return fail(ctx, MP_E_INVALID, "capacity too small");
+        memcpy(counts, ctx->pass_counts.data(), n * sizeof(int32_t));
+    }
+    return MP_OK;
+}
+
+extern "C" int mp_get_stats(mp_context* ctx, mp_stats* stats)
+{
+    if (!ctx || !stats) return MP_E_INVALID;
+    *stats = ctx->stats;
+    return MP_OK;
+}
+
+extern "C" int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* ms_per_frame)
+{
+    if (!ctx || n < 1) return MP_E_INVALID;
+    if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame to replay");
+    int rc;
+    if (!ctx->device_driven) {
+        if (use_graph) return fail(ctx, MP_E_INVALID, "graph replay needs a device-driven split method");
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        for (int i = 0; i < n; ++i) {
+            int64_t dropped = 0, maxp = 0;
+            uint32_t dt = 0;
+            HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->stream));
+            if ((rc = run_segment_split(ctx, dt, dropped, maxp))) return rc;
+            if ((rc = launch_back_end(ctx))) return rc;
+        }
+    } else if (use_graph) {
+        if (!ctx->graph) {
+            hipGraph_t g = nullptr;
+            HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            rc = launch_device_frame(ctx);
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) { if (g) hipGraphDestroy(g); return rc; }
+            HIPCHK(ctx, e);
+            e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            HIPCHK(ctx, e);
+        }
+        HIPCHK(ctx, hipGraphLaunch(ctx->graph, ctx->stream));  // warm-up
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        for (int i = 0; i < n; ++i) HIPCHK(ctx, hipGraphLaunch(ctx->graph, ctx->stream));
+    } else {
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        for (int i = 0; i < n; ++i)
+            if ((rc = launch_device_frame(ctx))) return rc;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    HIPCHK(ctx, hipMemcpy(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost));
+    if (ctx->h_fc->overflow) return fail(ctx, MP_E_STATE, "replay overflowed the draw list");
+    if (ms_per_frame) *ms_per_frame = (double)ms / n;
+    return MP_OK;
+}
+
+extern "C" int mp_replay_stages(mp_context* ctx, int32_t n, double stage_ms[4])
+{
+    if (!ctx || n < 1 || !stage_ms) return MP_E_INVALID;
+    if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame to replay");
+    if (!ctx->device_driven) return fail(ctx, MP_E_INVALID, "stage replay needs a device-driven split method");
+    std::vector<hipEvent_t> ev((size_t)n * 5, nullptr);
+    int rc = MP_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { rc = fail(ctx, MP_E_HIP, "hipEventCreate failed"); break; }
+    for (int i = 0; i < n && rc == MP_OK; ++i) rc = launch_device_frame(ctx, &ev[(size_t)i * 5]);
+    double acc[4] = { 0, 0, 0, 0 };
+    if (rc == MP_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, MP_E_HIP, "sync failed");
+    for (int i = 0; i < n && rc == MP_OK; ++i)
+        for (int k = 0; k < 4; ++k) {
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, ev[(size_t)i * 5 + k], ev[(size_t)i * 5 + k + 1]);
+            acc[k] += ms;
+        }
+    for (auto& e : ev)
+        if (e) hipEventDestroy(e);
+    if (rc) return rc;
+    for (int k = 0; k < 4; ++k) stage_ms[k] = acc[k] / n;
+    return MP_OK;
+}

@@ -63,4 +63,7 @@ hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           const RangeRec* canon, const DevObject* objs, const float* quad_store,
                           int patch_size, int write_rgba32, hipStream_t s);
 
+hipError_t launch_init_roots(int obj, uint32_t n, RangeRec* out, hipStream_t s);
+int split_persistent_occupancy(size_t lds_bytes);
+
 }  // namespace mp

@@ -34,6 +34,7 @@ __global__ void k_reset(FrameCounters* fc)
         fc->slow_blocks = 0;
         fc->max_stack = 0;
         fc->total_ranges = 0;
+        fc->atomics = 0;
     }
 }
 
@@ -483,6 +484,13 @@ MP_DEV int wave_max_i(int x)
     return x;
 }
 
+MP_DEV int wave_sum_i(int x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
 MP_DEV void normalize3(float& x, float& y, float& z)
 {
     const float inv = 1.0f / sqrtf(dot3(x, y, z, x, y, z));
@@ -540,7 +548,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     const int g0 = u + v * G1;
     const int sh = prm.subpixel_bits;
     const uint32_t total = fc->total_ranges;
-    uint32_t slow = 0;
+    uint32_t slow = 0, natom = 0;
 
     for (uint32_t r = blockIdx.x; r < total; r += gridDim.x) {
         const RangeRec rec = canon[r];
@@ -626,6 +634,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
                             if (in1 || in2) {
                                 const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
                                 atomicMin(&keys[fb_index(X, Y, gw)], key);
+                                ++natom;
                             }
                         }
                     }
@@ -650,6 +659,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
                                 if (in1 || in2) {
                                     const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
                                     atomicMin(&keys[fb_index(tx * 8 + x, ty * 8 + y, gw)], key);
+                                    ++natom;
                                 }
                             }
                         }
@@ -660,6 +670,8 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
         __syncthreads();
     }
     if (slow) atomicAdd(&fc->slow_blocks, slow);
+    natom = (uint32_t)wave_sum_i((int)natom);
+    if (lane == 0 && natom) atomicAdd(&fc->atomics, (unsigned long long)natom);
 }
 
 template <int N>
@@ -765,6 +777,36 @@ hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
     hipLaunchKernelGGL(k_resolve, dim3(bx, n_views), dim3(256), 0, s, views, canon, objs,
                        quad_store, patch_size * patch_size, write_rgba32);
     return hipGetLastError();
+}
+
+}  // namespace mp
+
+namespace mp {
+
+__global__ void k_init_roots(int obj, uint32_t n, RangeRec* out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    RangeRec r;
+    r.pid = i;
+    r.obj_depth = (uint32_t)obj;
+    r.umin = 0.0f; r.vmin = 0.0f; r.umax = 1.0f; r.vmax = 1.0f;
+    out[i] = r;
+}
+
+hipError_t launch_init_roots(int obj, uint32_t n, RangeRec* out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init_roots, dim3((n + 255) / 256), dim3(256), 0, s, obj, n, out);
+    return hipGetLastError();
+}
+
+int split_persistent_occupancy(size_t lds_bytes)
+{
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_split_persistent, 64, lds_bytes) != hipSuccess)
+        blocks = 1;
+    return blocks < 1 ? 1 : blocks;
 }
 
 }  // namespace mp

@@ -80,6 +80,7 @@ struct FrameCounters {
     uint32_t max_stack;          // largest LDS stack height seen
     uint32_t total_ranges;       // canonical ranges (written by the scan)
     uint32_t pad_;
+    unsigned long long atomics;  // (micropolygon, pixel) coverage events = key atomics
 };
 
 }  // namespace mp

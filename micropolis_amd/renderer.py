@@ -1,0 +1,185 @@
+"""Host-side mirror of the reference renderer interface over the HIP library.
+
+``Renderer`` follows Reyes::Renderer (src/Reyes/Renderer.h:29-45):
+prepare / are_patches_loaded / load_patches / draw_patches / finish, plus the
+framebuffer readback of Reyes::Framebuffer (src/Reyes/Framebuffer.cpp:107-124).
+Option names are the reference's reyes.options names.  Errors raise
+``MicropolisError`` with the library's message, as CL::Exception does in the
+reference (src/CL/Exception.h:28-29).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (MP_BEZIER, MP_GREGORY, METHODS, mp_projection, mp_range_record,
+                   mp_stats)
+
+BEZIER, GREGORY = MP_BEZIER, MP_GREGORY
+
+
+class MicropolisError(RuntimeError):
+    pass
+
+
+class Projection:
+    """Reyes::Projection(fovy, hither, viewport) (src/base/Projection.h:44)."""
+
+    def __init__(self, fovy, near, viewport):
+        self.fovy = float(fovy)
+        self.near = float(near)
+        self.viewport = (int(viewport[0]), int(viewport[1]))
+
+    def as_c(self):
+        return mp_projection(self.fovy, self.near, self.viewport[0], self.viewport[1])
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _colmajor(m):
+    m = np.asarray(m, dtype=np.float32)
+    if m.shape == (4, 4):
+        return np.ascontiguousarray(m.T.reshape(16))
+    return np.ascontiguousarray(m.reshape(16))
+
+
+class Renderer:
+    """Reyes::RendererCL over libmicropolis_hip.so."""
+
+    def __init__(self, **options):
+        lib = _lib.load()
+        cfg = _lib.default_config()
+        for key, val in options.items():
+            if key == "window_size":
+                cfg.window_width, cfg.window_height = int(val[0]), int(val[1])
+            elif key == "clear_color":
+                for i in range(4):
+                    cfg.clear_color[i] = float(val[i])
+            elif key == "bound_n_split_method":
+                cfg.bound_n_split_method = METHODS[val] if isinstance(val, str) else int(val)
+            elif hasattr(cfg, key):
+                setattr(cfg, key, type(getattr(cfg, key))(val))
+            else:
+                raise MicropolisError("unknown option %r" % key)
+        self.cfg = cfg
+        self._lib = lib
+        ctx = lib.mp_create(C.byref(cfg))
+        if not ctx:
+            raise MicropolisError(lib.mp_create_error().decode())
+        self._ctx = ctx
+        self._views = [(cfg.window_width, cfg.window_height)]
+
+    # -- helpers --
+    def _check(self, rc):
+        if rc != 0:
+            raise MicropolisError(self._lib.mp_last_error(self._ctx).decode())
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.mp_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- Reyes::Renderer --
+    def prepare(self):
+        self._check(self._lib.mp_prepare(self._ctx))
+        self._views = [(self.cfg.window_width, self.cfg.window_height)]
+
+    def finish(self):
+        self._check(self._lib.mp_finish(self._ctx))
+
+    def are_patches_loaded(self, handle):
+        return bool(self._lib.mp_are_patches_loaded(self._ctx, int(handle)))
+
+    def load_patches(self, handle, patch_data, patch_type=BEZIER):
+        pts = np.ascontiguousarray(np.asarray(patch_data, dtype=np.float32).reshape(-1, 3))
+        self._check(self._lib.mp_load_patches(self._ctx, int(handle), _fptr(pts), pts.shape[0],
+                                              int(patch_type)))
+
+    def unload_patches(self, handle):
+        self._check(self._lib.mp_unload_patches(self._ctx, int(handle)))
+
+    def draw_patches(self, handle, matrix, projection, color):
+        mv = _colmajor(matrix)
+        col = np.ascontiguousarray(np.asarray(color, dtype=np.float32).reshape(4))
+        pr = projection.as_c()
+        self._check(self._lib.mp_draw_patches(self._ctx, int(handle), _fptr(mv), C.byref(pr),
+                                              _fptr(col)))
+
+    def set_displacement_amplitude(self, amplitude):
+        self._check(self._lib.mp_set_displacement_amplitude(self._ctx, float(amplitude)))
+
+    # -- batching --
+    def batch_begin(self, projections):
+        arr = (mp_projection * len(projections))(*[p.as_c() for p in projections])
+        self._check(self._lib.mp_batch_begin(self._ctx, len(projections), arr))
+        self._views = [p.viewport for p in projections]
+
+    def batch_draw(self, view, handle, matrix, color):
+        mv = _colmajor(matrix)
+        col = np.ascontiguousarray(np.asarray(color, dtype=np.float32).reshape(4))
+        self._check(self._lib.mp_batch_draw(self._ctx, int(view), int(handle), _fptr(mv), _fptr(col)))
+
+    def batch_finish(self):
+        self._check(self._lib.mp_batch_finish(self._ctx))
+
+    # -- readback --
+    def read_framebuffer(self, view=0, rgba32=False):
+        w, h = self._views[view]
+        out = {
+            "rgba8": np.empty((h, w, 4), np.uint8),
+            "depth": np.empty((h, w), np.float32),
+            "ids": np.empty((h, w), np.uint32),
+        }
+        p32 = None
+        if rgba32:
+            out["rgba32"] = np.empty((h, w, 4), np.float32)
+            p32 = out["rgba32"].ctypes.data
+        self._check(self._lib.mp_batch_read(self._ctx, int(view), out["rgba8"].ctypes.data,
+                                            out["depth"].ctypes.data, out["ids"].ctypes.data, p32))
+        return out
+
+    def ranges(self):
+        n = C.c_int64()
+        self._check(self._lib.mp_read_ranges(self._ctx, None, 0, C.byref(n)))
+        recs = (mp_range_record * max(1, n.value))()
+        self._check(self._lib.mp_read_ranges(self._ctx, recs, n.value, C.byref(n)))
+        a = np.frombuffer(recs, dtype=np.dtype([("pid", "<i4"), ("object", "<i4"), ("depth", "<i4"),
+                                                ("umin", "<f4"), ("vmin", "<f4"),
+                                                ("umax", "<f4"), ("vmax", "<f4")]))
+        return a[: n.value].copy()
+
+    def pass_counts(self):
+        n = C.c_int32()
+        self._check(self._lib.mp_read_pass_counts(self._ctx, None, 0, C.byref(n)))
+        arr = (C.c_int32 * max(1, n.value))()
+        self._check(self._lib.mp_read_pass_counts(self._ctx, arr, n.value, C.byref(n)))
+        return np.array(arr[: n.value], dtype=np.int32)
+
+    def stats(self):
+        st = mp_stats()
+        self._check(self._lib.mp_get_stats(self._ctx, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in mp_stats._fields_}
+
+    def replay_stages(self, n):
+        ms = (C.c_double * 4)()
+        self._check(self._lib.mp_replay_stages(self._ctx, int(n), ms))
+        return dict(zip(("split", "order", "raster", "resolve"), list(ms)))
+
+    def replay(self, n, use_graph=False):
+        ms = C.c_double()
+        self._check(self._lib.mp_replay(self._ctx, int(n), int(bool(use_graph)), C.byref(ms)))
+        return ms.value
