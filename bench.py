@@ -21,6 +21,43 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measure_batch(steps, use_graph):
+    """64 uneven scenes (640x480, log-uniform 100..20000 random patches): one
+    batched frame against 64 plain frames, both replayed device-resident."""
+    from micropolis_amd import Renderer, scenes
+
+    w = scenes.c3_batch()
+    r = Renderer(**w.options)
+    for v in w.views:
+        o = v.objects[0]
+        r.load_patches(o.handle, o.patches, o.patch_type)
+    plain_ms = 0.0
+    mpolys = 0
+    for v in w.views:
+        o = v.objects[0]
+        r.prepare()
+        r.draw_patches(o.handle, o.matrix, v.projection, o.color)
+        r.finish()
+        mpolys += r.stats()["micropolygons"]
+        r.replay(1, use_graph=use_graph)
+        plain_ms += r.replay(steps, use_graph=use_graph)
+    r.batch_begin([v.projection for v in w.views])
+    for vi, v in enumerate(w.views):
+        o = v.objects[0]
+        r.batch_draw(vi, o.handle, o.matrix, o.color)
+    r.batch_finish()
+    st = r.stats()
+    r.replay(1, use_graph=use_graph)
+    batch_ms = r.replay(steps, use_graph=use_graph)
+    r.close()
+    print("batch: 64 plain frames %.3f ms, one batched frame %.3f ms" % (plain_ms, batch_ms),
+          file=sys.stderr, flush=True)
+    return {"workload": w.name, "scenes": len(w.views), "patches": w.patch_count,
+            "micropolygons": int(st["micropolygons"]), "plain_64_frames_ms": round(plain_ms, 4),
+            "batched_ms": round(batch_ms, 4), "speedup": round(plain_ms / batch_ms, 3),
+            "same_micropolygons": int(mpolys) == int(st["micropolygons"])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -28,6 +65,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -83,6 +121,8 @@ def main():
     raster_ms = stages["raster"]
     achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
 
+    batch = None if args.no_batch else measure_batch(max(3, min(args.steps, 10)), use_graph)
+
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import pyoracle
@@ -122,6 +162,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "kernel": "k_raster<16>", "alg_bytes": int(alg_bytes)},
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     r.close()
     print(json.dumps(line), flush=True)

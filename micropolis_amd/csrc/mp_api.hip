@@ -14,6 +14,7 @@
 
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -88,6 +89,10 @@ struct mp_context {
     int32_t* d_queue_cnt = nullptr; int queue_groups_cap = 0;
     RangeRec* d_shared_q = nullptr; size_t shared_entries = 0;
     int32_t* d_shared_cnt = nullptr;
+    RangeRec* d_chunks = nullptr; uint32_t* d_chunk_count = nullptr; uint32_t* d_chunk_ready = nullptr;
+    uint32_t chunk_cap = 0;
+    ShareState* d_share = nullptr;
+    unsigned long long* d_keys = nullptr;
 
     // segment engine (BREADTH / BOUNDED)
     RangeRec* d_seg_a = nullptr; size_t seg_a_cap = 0;
@@ -115,6 +120,9 @@ struct mp_context {
     bool frame_valid = false;
     bool device_driven = false;  // the last frame can be replayed without host syncs
     hipGraphExec_t graph = nullptr;
+    bool work_sharing = false;   // spill queue between split groups (off: measured slower)
+    int spill_at = 192;
+    int linger = 0;
 };
 
 #define HIPCHK(ctx, expr)                                                                  \
@@ -309,8 +317,20 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
         return nullptr;
     }
     ctx->num_cus = prop.multiProcessorCount;
-    ctx->raster_grid_size = raster_grid(cfg->reyes_patch_size, ctx->num_cus);
+    // hashing the lattice per use measured faster than the per-range LDS cache
+    ctx->prm.noise_cache = 0;
+    if (const char* e = getenv("MP_NOISE_CACHE")) ctx->prm.noise_cache = atoi(e);
+    ctx->raster_grid_size = raster_grid(cfg->reyes_patch_size, ctx->num_cus, ctx->prm);
+    // LDS ring stack of the persistent split, depth-bounded: 64 * (max_split_depth + 2)
+    // entries.  Fewer resident groups that own their SIMD finish the deepest
+    // split trees sooner than more groups sharing it (measured on the torus frame).
     ctx->stack_cap = 64 * (cfg->max_split_depth + 2);
+    // tuning knobs (defaults chosen by measurement; see the autotuner)
+    if (const char* e = getenv("MP_WORK_SHARING")) ctx->work_sharing = atoi(e) != 0;
+    if (const char* e = getenv("MP_SPILL_AT")) ctx->spill_at = atoi(e);
+    if (const char* e = getenv("MP_LINGER")) ctx->linger = atoi(e);
+    if (const char* e = getenv("MP_STACK_CAP")) ctx->stack_cap = std::max(256, atoi(e));
+    if (const char* e = getenv("MP_RASTER_PER_CU")) ctx->raster_grid_size = ctx->num_cus * std::max(1, atoi(e));
     memset(&ctx->stats, 0, sizeof(ctx->stats));
     return ctx;
 }
@@ -333,6 +353,8 @@ extern "C" void mp_destroy(mp_context* ctx)
     hipFree(ctx->d_fc); hipHostFree(ctx->h_fc);
     hipFree(ctx->d_totals); hipHostFree(ctx->h_totals);
     hipFree(ctx->d_queue); hipFree(ctx->d_queue_cnt); hipFree(ctx->d_shared_q); hipFree(ctx->d_shared_cnt);
+    hipFree(ctx->d_chunks); hipFree(ctx->d_chunk_count); hipFree(ctx->d_chunk_ready); hipFree(ctx->d_share);
+    hipFree(ctx->d_keys);
     hipFree(ctx->d_seg_a); hipFree(ctx->d_seg_b); hipFree(ctx->d_flags); hipFree(ctx->d_blk);
     hipFree(ctx->d_draw); hipFree(ctx->d_bucket); hipFree(ctx->d_canon);
     hipFree(ctx->d_counts); hipFree(ctx->d_offsets); hipFree(ctx->d_cursor);
@@ -481,7 +503,7 @@ static int ensure_lists(mp_context* ctx, uint32_t cap)
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     int rc;
     if ((rc = dev_alloc(ctx, &ctx->d_draw, cap)) || (rc = dev_alloc(ctx, &ctx->d_bucket, cap)) ||
-        (rc = dev_alloc(ctx, &ctx->d_canon, cap)))
+        (rc = dev_alloc(ctx, &ctx->d_canon, cap)) || (rc = dev_alloc(ctx, &ctx->d_keys, cap)))
         return rc;
     const size_t q = (size_t)cap * ctx->cfg.reyes_patch_size * ctx->cfg.reyes_patch_size;
     if ((rc = dev_alloc(ctx, &ctx->d_quad, q))) return rc;
@@ -500,7 +522,9 @@ static int grow_lists(mp_context* ctx, uint32_t need, uint32_t keep)
     int rc;
     if ((rc = dev_grow(ctx, &ctx->d_draw, &cap, need, keep))) return rc;
     const uint32_t ncap = (uint32_t)cap;
-    if ((rc = dev_alloc(ctx, &ctx->d_bucket, ncap)) || (rc = dev_alloc(ctx, &ctx->d_canon, ncap))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_bucket, ncap)) || (rc = dev_alloc(ctx, &ctx->d_canon, ncap)) ||
+        (rc = dev_alloc(ctx, &ctx->d_keys, ncap)))
+        return rc;
     const size_t q = (size_t)ncap * ctx->cfg.reyes_patch_size * ctx->cfg.reyes_patch_size;
     if ((rc = dev_alloc(ctx, &ctx->d_quad, q))) return rc;
     ctx->quad_cap = q;
@@ -531,6 +555,7 @@ static CanonBuffers canon_buffers(mp_context* ctx)
     cb.draw = ctx->d_draw;
     cb.bucket = ctx->d_bucket;
     cb.canon = ctx->d_canon;
+    cb.keys = ctx->d_keys;
     cb.cap = ctx->list_cap;
     cb.counts = ctx->d_counts;
     cb.offsets = ctx->d_offsets;
@@ -637,6 +662,23 @@ static int ensure_queues(mp_context* ctx)
         invalidate_graph(ctx);
     }
     if (!ctx->d_shared_cnt && (rc = dev_alloc(ctx, &ctx->d_shared_cnt, 1))) return rc;
+    // spilled-chunk pool for work sharing: enough for every range of a
+    // frame four times over; running out only disables further spills
+    const uint32_t want_chunks = std::max<uint32_t>(4096, (ctx->total_roots * 64u) / 64u + 4096);
+    if (want_chunks > ctx->chunk_cap) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = dev_alloc(ctx, &ctx->d_chunks, (size_t)want_chunks * 64)) ||
+            (rc = dev_alloc(ctx, &ctx->d_chunk_count, want_chunks)) ||
+            (rc = dev_alloc(ctx, &ctx->d_chunk_ready, want_chunks)))
+            return rc;
+        HIPCHK(ctx, launch_fill_u32(ctx->d_chunk_ready, want_chunks, 0u, ctx->stream));
+        ctx->chunk_cap = want_chunks;
+        invalidate_graph(ctx);
+    }
+    if (!ctx->d_share) {
+        if ((rc = dev_alloc(ctx, &ctx->d_share, 1))) return rc;
+        HIPCHK(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->d_share), 4, 0u, ctx->stream));
+    }
     if (balanced && ctx->total_roots > ctx->shared_entries) {
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         if ((rc = dev_alloc(ctx, &ctx->d_shared_q, ctx->total_roots))) return rc;
@@ -657,6 +699,13 @@ static SplitQueues split_queues(mp_context* ctx)
     q.shared_cnt = ctx->d_shared_cnt;
     q.n_groups = ctx->split_groups;
     q.stack_cap = ctx->stack_cap;
+    q.chunks = ctx->d_chunks;
+    q.chunk_count = ctx->d_chunk_count;
+    q.chunk_ready = ctx->d_chunk_ready;
+    q.chunk_cap = ctx->work_sharing ? ctx->chunk_cap : 0u;
+    q.spill_at = ctx->spill_at;
+    q.linger = ctx->linger;
+    q.share = ctx->d_share;
     return q;
 }
 
@@ -685,7 +734,7 @@ static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
 {
     const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
-    HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->stream));
+    HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
     SplitQueues q = split_queues(ctx);
     if (ctx->total_roots) {
         HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
@@ -781,6 +830,7 @@ static int execute_frame(mp_context* ctx)
 {
     int rc;
     ctx->frame_valid = false;
+    invalidate_graph(ctx);  // a captured graph bakes the previous frame's launch sizes
     if ((rc = ensure_views(ctx))) return rc;
     if ((rc = upload_tables(ctx))) return rc;
     if ((rc = ensure_gpids(ctx, ctx->n_gpids))) return rc;
@@ -797,11 +847,18 @@ static int execute_frame(mp_context* ctx)
             if ((rc = launch_device_frame(ctx))) return rc;
             HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-            if (ctx->h_fc->overflow & 2u) return fail(ctx, MP_E_INVALID, "split stack overflow");
-            if (!(ctx->h_fc->overflow & 1u)) break;
-            // draw list too small: grow and render the frame again
+            const uint32_t ovf = ctx->h_fc->overflow;
+            if (ovf & 4u) return fail(ctx, MP_E_HIP, "split work sharing timed out");
+            if (!(ovf & 3u)) break;
             ++relaunches;
-            if ((rc = ensure_lists(ctx, ctx->list_cap * 2))) return rc;
+            if (ovf & 2u) {
+                const int full = 64 * (ctx->cfg.max_split_depth + 2);
+                if (ctx->stack_cap >= full) return fail(ctx, MP_E_INVALID, "split stack overflow");
+                ctx->stack_cap = full;   // deeper LDS stack; fewer resident groups
+                if ((rc = ensure_queues(ctx))) return rc;
+            }
+            // draw list too small: grow and render the frame again
+            if ((ovf & 1u) && (rc = ensure_lists(ctx, ctx->list_cap * 2))) return rc;
             HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         }
         draw_total = ctx->h_fc->total_ranges;
@@ -814,7 +871,7 @@ static int execute_frame(mp_context* ctx)
         } while (left > 0);
         max_patches = ctx->h_fc->max_stack;
     } else {
-        HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->stream));
+        HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
         if ((rc = run_segment_split(ctx, draw_total, dropped, max_patches))) return rc;
         if ((rc = launch_back_end(ctx))) return rc;
         HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
@@ -974,7 +1031,7 @@ extern "C" int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* 
         for (int i = 0; i < n; ++i) {
             int64_t dropped = 0, maxp = 0;
             uint32_t dt = 0;
-            HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->stream));
+            HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
             if ((rc = run_segment_split(ctx, dt, dropped, maxp))) return rc;
             if ((rc = launch_back_end(ctx))) return rc;
         }

@@ -7,6 +7,13 @@
 
 namespace mp {
 
+// Work-sharing state of the persistent split (reset by launch_reset).
+struct ShareState {
+    uint32_t head, tail;     // chunk queue: popped / reserved chunk counts
+    uint32_t active;         // work-groups currently holding work
+    uint32_t epoch;          // frame stamp of published chunks
+};
+
 struct SplitQueues {
     RangeRec* queue;         // private LIFO queues, stride entries each
     int32_t* queue_cnt;      // entries per private queue
@@ -14,13 +21,21 @@ struct SplitQueues {
     RangeRec* shared_q;      // BALANCED: one shared LIFO queue
     int32_t* shared_cnt;
     int32_t n_groups;        // persistent work-groups
-    int32_t stack_cap;       // LDS stack entries per work-group
+    int32_t stack_cap;       // LDS ring-stack entries per work-group
+    RangeRec* chunks;        // spilled stack chunks, 64 ranges each
+    uint32_t* chunk_count;   // ranges in each chunk
+    uint32_t* chunk_ready;   // epoch stamp once a chunk is published
+    uint32_t chunk_cap;      // 0 disables work sharing
+    int32_t spill_at;        // spill 64 ranges when the stack exceeds this
+    int32_t linger;          // idle groups wait (bounded) for spilled chunks
+    ShareState* share;
 };
 
 struct CanonBuffers {
     RangeRec* draw;          // DRAW records in emission order
     RangeRec* bucket;        // grouped by global patch index
     RangeRec* canon;         // canonical order
+    unsigned long long* keys;  // canonical keys of the bucketed records
     uint32_t cap;            // capacity of the three lists
     uint32_t* counts;        // per global patch: DRAW count
     uint32_t* offsets;       // exclusive scan of counts (n_gpids + 1)
@@ -28,7 +43,7 @@ struct CanonBuffers {
     uint32_t n_gpids;
 };
 
-hipError_t launch_reset(FrameCounters* fc, hipStream_t s);
+hipError_t launch_reset(FrameCounters* fc, ShareState* share, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
@@ -58,7 +73,7 @@ hipError_t launch_canonicalize(const DevObject* objs, DevView* views, int n_view
 hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject* objs,
                          const DevView* views, const DevParams& prm, FrameCounters* fc,
                          float* quad_store, int grid, hipStream_t s);
-int raster_grid(int patch_size, int num_cus);
+int raster_grid(int patch_size, int num_cus, const DevParams& prm);
 hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           const RangeRec* canon, const DevObject* objs, const float* quad_store,
                           int patch_size, int write_rgba32, hipStream_t s);

@@ -25,8 +25,14 @@ namespace mp {
 // ------------------------------------------------------------------------
 // small utilities
 
-__global__ void k_reset(FrameCounters* fc)
+__global__ void k_reset(FrameCounters* fc, ShareState* share)
 {
+    if (threadIdx.x == 0 && share) {
+        share->head = 0;
+        share->tail = 0;
+        share->active = 0;
+        share->epoch = share->epoch + 1u;
+    }
     if (threadIdx.x == 0) {
         fc->draw_count = 0;
         fc->overflow = 0;
@@ -35,6 +41,8 @@ __global__ void k_reset(FrameCounters* fc)
         fc->max_stack = 0;
         fc->total_ranges = 0;
         fc->atomics = 0;
+        fc->raster_next = 0;
+        fc->uncached_octaves = 0;
     }
 }
 
@@ -52,9 +60,9 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
     for (; i < n; i += st) p[i] = v;
 }
 
-hipError_t launch_reset(FrameCounters* fc, hipStream_t s)
+hipError_t launch_reset(FrameCounters* fc, ShareState* share, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, fc);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, fc, share);
     return hipGetLastError();
 }
 
@@ -130,26 +138,78 @@ hipError_t launch_init_queues(const DevObject* objs, int n_objs, uint32_t total_
     return hipGetLastError();
 }
 
-// One wave per work-group.  Pops up to 64 ranges (private queue, then the
-// shared queue, then the LDS stack), bounds them, pushes split children on
-// the LDS stack (ballot rank) and appends DRAW ranges to the frame's draw
-// list (one atomic per wave).  Exits when every source is empty.
+// Per-wave aggregated atomicAdd(&counts[key], 1) over the lanes with pred.
+MP_DEV void wave_count_add(uint32_t* counts, uint32_t key, bool pred)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t m = __ballot(pred);
+    while (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        const uint32_t lk = __shfl(key, leader);
+        const uint64_t same = __ballot(pred && key == lk) & m;
+        if (lane == leader) atomicAdd(&counts[lk], (uint32_t)__popcll(same));
+        m &= ~same;
+    }
+}
+
+constexpr uint32_t kFlagSpin = 1u << 20;     // bound on waiting for a reserved chunk
+constexpr uint32_t kLingerPolls = 1u << 14;  // bound on waiting for spilled work
+
+// Lane 0 only: reserve the next published chunk; -1 if none.  The wait for
+// the publishing store is bounded: the publisher is a running wave that
+// writes the chunk right after reserving it.
+MP_DEV int try_pop_chunk(const SplitQueues& q, uint32_t epoch, FrameCounters* fc)
+{
+    ShareState* st = q.share;
+    for (;;) {
+        const uint32_t h = __hip_atomic_load(&st->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t t = __hip_atomic_load(&st->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t = min(t, q.chunk_cap);
+        if (h >= t) return -1;
+        if (atomicCAS(&st->head, h, h + 1) == h) {
+            for (uint32_t spin = 0;; ++spin) {
+                if (__hip_atomic_load(&q.chunk_ready[h], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+                    return (int)h;
+                if (spin > kFlagSpin) {
+                    atomicOr(&fc->overflow, 4u);
+                    return -1;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+    }
+}
+
+// One wave per work-group, following kernels/bound_n_split_local.cl: pop up
+// to 64 ranges (private queue, then the shared BALANCED queue, then spilled
+// chunks, then the LDS stack), bound them, push split children on the LDS
+// ring stack (ballot rank) and append DRAW ranges to the frame's draw list
+// (one atomic per wave).  A stack that grows past spill_at publishes its 64
+// oldest (shallowest) ranges as a chunk that idle work-groups pick up.  A
+// work-group with no work keeps polling for chunks while others are active
+// (bounded); every chunk is popped because its publisher drains the chunk
+// queue before it exits.
 __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __restrict__ objs,
                                                          const DevView* __restrict__ views,
                                                          DevParams prm, SplitQueues q,
                                                          CanonBuffers cb, FrameCounters* fc,
                                                          int balanced)
 {
-    extern __shared__ RangeRec stack[];
+    extern __shared__ RangeRec ring[];
     const int lane = threadIdx.x;
     const int wid = blockIdx.x;
+    const int cap = q.stack_cap;
     RangeRec* myq = q.queue + (int64_t)wid * q.stride;
-    int sh = 0;
+    int base = 0, sh = 0;
     int qcnt = q.queue_cnt[wid];
     uint32_t nbounds = 0, maxsh = 0;
+    const uint32_t epoch = q.share ? q.share->epoch : 0u;
+    const bool sharing = q.chunk_cap > 0;
+    if (sharing && lane == 0) atomicAdd(&q.share->active, 1u);
+    bool active = true;
 
     for (;;) {
-        int cnt = 0, start = 0, scnt, src = 0;
+        int cnt = 0, start = 0, scnt = 0, src = 0;
         if (sh < 64) {
             const int want = 64 - sh;
             if (qcnt > 0) {
@@ -164,20 +224,65 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
                 start = top - cnt;
                 src = 1;
             }
+            if (cnt == 0 && sh == 0 && sharing) {
+                int c = -1;
+                if (lane == 0) c = try_pop_chunk(q, epoch, fc);
+                c = __shfl(c, 0);
+                if (c >= 0) {
+                    cnt = (int)q.chunk_count[c];
+                    start = c * 64;
+                    src = 2;
+                }
+            }
             scnt = min(sh, 64 - cnt);
         } else {
             scnt = 64;
         }
         sh -= scnt;
-        if (cnt == 0 && scnt == 0) break;
+        if (cnt == 0 && scnt == 0) {
+            if (!sharing || !q.linger) {
+                if (sharing && active && lane == 0) atomicSub(&q.share->active, 1u);
+                active = false;
+                break;
+            }
+            // idle: wait (bounded) for chunks while other work-groups hold work
+            if (active && lane == 0) atomicSub(&q.share->active, 1u);
+            active = false;
+            int c = -1;
+            if (lane == 0) {
+                for (uint32_t it = 0; it < kLingerPolls; ++it) {
+                    const uint32_t h = __hip_atomic_load(&q.share->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint32_t t = min(__hip_atomic_load(&q.share->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), q.chunk_cap);
+                    if (h < t) {
+                        atomicAdd(&q.share->active, 1u);
+                        c = try_pop_chunk(q, epoch, fc);
+                        if (c >= 0) break;
+                        atomicSub(&q.share->active, 1u);
+                    } else if (__hip_atomic_load(&q.share->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                        break;
+                    }
+                    // back off so that idle groups do not flood the shared lines
+                    if (it < 8) __builtin_amdgcn_s_sleep(8);
+                    else __builtin_amdgcn_s_sleep(64);
+                }
+            }
+            c = __shfl(c, 0);
+            if (c < 0) break;
+            active = true;
+            cnt = (int)q.chunk_count[c];
+            start = c * 64;
+            src = 2;
+            scnt = 0;
+        }
 
         RangeRec rec;
         bool occ = false;
         if (lane < cnt) {
-            rec = src ? q.shared_q[start + lane] : myq[start + lane];
+            rec = (src == 2) ? q.chunks[start + lane] : (src == 1 ? q.shared_q[start + lane] : myq[start + lane]);
             occ = true;
         } else if (lane < cnt + scnt) {
-            rec = stack[sh + lane - cnt];
+            int k = base + sh + lane - cnt;
+            rec = ring[k >= cap ? k - cap : k];
             occ = true;
         }
         __syncthreads();
@@ -187,7 +292,7 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
 
         const uint64_t sb = __ballot(flags & 2);
         const int ns = __popcll(sb);
-        if (sh + 2 * ns > q.stack_cap) {
+        if (sh + 2 * ns > cap) {
             if (lane == 0) atomicOr(&fc->overflow, 2u);
             break;
         }
@@ -195,8 +300,12 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
             const uint32_t rk = lanes_below(sb);
             RangeRec c0, c1;
             split_children(rec, flags, c0, c1);
-            stack[sh + 2 * rk] = c0;
-            stack[sh + 2 * rk + 1] = c1;
+            int k0 = base + sh + 2 * (int)rk;
+            if (k0 >= cap) k0 -= cap;
+            int k1 = k0 + 1;
+            if (k1 >= cap) k1 -= cap;
+            ring[k0] = c0;
+            ring[k1] = c1;
         }
         sh += 2 * ns;
         maxsh = max(maxsh, (uint32_t)sh);
@@ -204,24 +313,40 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
         const uint64_t db = __ballot(flags & 1);
         const int nd = __popcll(db);
         if (nd) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&fc->draw_count, (uint32_t)nd);
-            base = __shfl(base, 0);
-            if (flags & 1) {
-                const uint32_t idx = base + lanes_below(db);
-                if (idx < cb.cap) {
-                    cb.draw[idx] = rec;
-                    atomicAdd(&cb.counts[gpid_of(objs, rec)], 1u);
-                }
-            }
-            if (base + (uint32_t)nd > cb.cap) {
+            uint32_t dbase = 0;
+            if (lane == 0) dbase = atomicAdd(&fc->draw_count, (uint32_t)nd);
+            dbase = __shfl(dbase, 0);
+            const uint32_t idx = dbase + lanes_below(db);
+            const bool put = (flags & 1) && idx < cb.cap;
+            if (put) cb.draw[idx] = rec;
+            wave_count_add(cb.counts, put ? gpid_of(objs, rec) : 0u, put);
+            if (dbase + (uint32_t)nd > cb.cap) {
                 if (lane == 0) atomicOr(&fc->overflow, 1u);
                 break;
             }
         }
         __syncthreads();
+
+        // publish the 64 oldest stack entries while the stack is deep
+        while (sharing && sh > q.spill_at) {
+            uint32_t idx = 0;
+            if (lane == 0) idx = atomicAdd(&q.share->tail, 1u);
+            idx = __shfl(idx, 0);
+            if (idx >= q.chunk_cap) break;
+            int k = base + lane;
+            if (k >= cap) k -= cap;
+            q.chunks[(size_t)idx * 64 + lane] = ring[k];
+            if (lane == 0) q.chunk_count[idx] = 64u;
+            __threadfence();
+            __syncthreads();
+            if (lane == 0) __hip_atomic_store(&q.chunk_ready[idx], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            base += 64;
+            if (base >= cap) base -= cap;
+            sh -= 64;
+        }
     }
     if (lane == 0) {
+        if (sharing && active) atomicSub(&q.share->active, 1u);
         atomicAdd(&fc->bounds, (unsigned long long)nbounds);
         atomicMax(&fc->max_stack, maxsh);
     }
@@ -430,11 +555,34 @@ __global__ __launch_bounds__(256) void k_scatter(const DevObject* __restrict__ o
                                                  const FrameCounters* __restrict__ fc)
 {
     const uint32_t total = fc->total_ranges;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
-        const RangeRec r = cb.draw[i];
-        const uint32_t g = gpid_of(objs, r);
-        const uint32_t slot = cb.offsets[g] + atomicAdd(&cb.cursor[g], 1u);
-        cb.bucket[slot] = r;
+    const int lane = threadIdx.x & 63;
+    const uint32_t stride = gridDim.x * 256;
+    const uint32_t lim = (total + 255u) & ~255u;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < lim; i += stride) {
+        const bool valid = i < total;
+        RangeRec r;
+        uint32_t g = 0;
+        if (valid) {
+            r = cb.draw[i];
+            g = gpid_of(objs, r);
+        }
+        // wave-aggregated bucket cursors: one atomic per (wave, patch)
+        uint64_t m = __ballot(valid);
+        uint32_t slot = 0;
+        while (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            const uint32_t lg = __shfl(g, leader);
+            const uint64_t same = __ballot(valid && g == lg) & m;
+            uint32_t b = 0;
+            if (lane == leader) b = cb.offsets[lg] + atomicAdd(&cb.cursor[lg], (uint32_t)__popcll(same));
+            b = __shfl(b, leader);
+            if ((same >> lane) & 1ull) slot = b + lanes_below(same);
+            m &= ~same;
+        }
+        if (valid) {
+            cb.bucket[slot] = r;
+            cb.keys[slot] = canonical_key(r);
+        }
     }
 }
 
@@ -446,9 +594,14 @@ __global__ __launch_bounds__(256) void k_rank(const DevObject* __restrict__ objs
         const RangeRec r = cb.bucket[j];
         const uint32_t g = gpid_of(objs, r);
         const uint32_t lo = cb.offsets[g], hi = cb.offsets[g + 1];
-        const uint64_t key = canonical_key(r);
+        const unsigned long long key = cb.keys[j];
         uint32_t rank = 0;
-        for (uint32_t k = lo; k < hi; ++k) rank += (canonical_key(cb.bucket[k]) < key) ? 1u : 0u;
+        uint32_t k = lo;
+        for (; k + 4 <= hi; k += 4) {
+            const unsigned long long a = cb.keys[k], b = cb.keys[k + 1], c = cb.keys[k + 2], d = cb.keys[k + 3];
+            rank += (a < key) + (b < key) + (c < key) + (d < key);
+        }
+        for (; k < hi; ++k) rank += (cb.keys[k] < key) ? 1u : 0u;
         cb.canon[lo + rank] = r;
     }
 }
@@ -525,7 +678,12 @@ MP_DEV float quad_lambert(const float4& p0, const float4& p1, const float4& p2, 
     return (spec != spec) ? spec : s;
 }
 
-template <int N>
+constexpr int kCachedOctaves = 4;
+constexpr int kCacheEntries = 512;
+
+// MODE 0: no displacement; 1: displacement, lattice values hashed per use;
+// 2: displacement with the per-range LDS lattice cache.
+template <int N, int MODE>
 __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ canon,
                                                   const DevObject* __restrict__ objs,
                                                   const DevView* __restrict__ views,
@@ -540,6 +698,10 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     __shared__ float4 s_pos[G];
     __shared__ int2 s_pxl[G];
     __shared__ float s_dep[G];
+    __shared__ float4 s_nrm[G];
+    __shared__ float s_box[Q / 64][6];
+    __shared__ int s_geo[kCachedOctaves][6];
+    __shared__ float s_lat[kCachedOctaves][kCacheEntries];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -548,9 +710,14 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     const int g0 = u + v * G1;
     const int sh = prm.subpixel_bits;
     const uint32_t total = fc->total_ranges;
-    uint32_t slow = 0, natom = 0;
+    uint32_t slow = 0, natom = 0, uncached = 0;
+    __shared__ uint32_t s_next;
 
-    for (uint32_t r = blockIdx.x; r < total; r += gridDim.x) {
+    for (;;) {
+        if (tid == 0) s_next = atomicAdd(&fc->raster_next, 1u);
+        __syncthreads();
+        const uint32_t r = s_next;
+        if (r >= total) break;
         const RangeRec rec = canon[r];
         const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
         const DevView& vw = views[ob.view];
@@ -559,20 +726,118 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
         __syncthreads();
 
         // ---- dice (dice.cl:38-79) ----
-        for (int g = tid; g < G; g += Q) {
-            const int nu = g % G1, nv = g / G1;
-            const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
-            const float uu = rec.umin + (rec.umax - rec.umin) * tu;
-            const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
-            float P[4], e[4], q[4];
-            displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
-            mat4_mul(ob.mv, P, e);
-            mat4_mul(vw.proj, e, q);
-            s_pos[g] = make_float4(e[0], e[1], e[2], e[3]);
-            const int32_t px = f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2));
-            const int32_t py = f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2));
-            s_pxl[g] = make_int2(px, py);
-            s_dep[g] = q[2] / q[3];
+        if (MODE != 2 || ob.type != MP_TYPE_BEZIER) {
+            for (int g = tid; g < G; g += Q) {
+                const int nu = g % G1, nv = g / G1;
+                const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
+                const float uu = rec.umin + (rec.umax - rec.umin) * tu;
+                const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
+                float P[4], e[4], q[4];
+                if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
+                else eval_patch(s_cp, ob.type, uu, vv, P);
+                mat4_mul(ob.mv, P, e);
+                mat4_mul(vw.proj, e, q);
+                s_pos[g] = make_float4(e[0], e[1], e[2], e[3]);
+                const int32_t px = f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2));
+                const int32_t py = f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2));
+                s_pxl[g] = make_int2(px, py);
+                s_dep[g] = q[2] / q[3];
+            }
+        } else {
+            // displaced: surface point and normal, then the value noise with
+            // each octave's lattice values hashed once per range into LDS
+            float bx0 = INFINITY, by0 = INFINITY, bz0 = INFINITY;
+            float bx1 = -INFINITY, by1 = -INFINITY, bz1 = -INFINITY;
+            for (int g = tid; g < G; g += Q) {
+                const int nu = g % G1, nv = g / G1;
+                const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
+                const float uu = rec.umin + (rec.umax - rec.umin) * tu;
+                const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
+                float P[4], n[3];
+                eval_bezier(s_cp, uu, vv, P);
+                bezier_unit_normal(s_cp, uu, vv, n);
+                s_pos[g] = make_float4(P[0], P[1], P[2], P[3]);
+                s_nrm[g] = make_float4(n[0], n[1], n[2], 0.0f);
+                bx0 = fminf(bx0, P[0]); by0 = fminf(by0, P[1]); bz0 = fminf(bz0, P[2]);
+                bx1 = fmaxf(bx1, P[0]); by1 = fmaxf(by1, P[1]); bz1 = fmaxf(bz1, P[2]);
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                bx0 = fminf(bx0, __shfl_xor(bx0, o)); by0 = fminf(by0, __shfl_xor(by0, o));
+                bz0 = fminf(bz0, __shfl_xor(bz0, o)); bx1 = fmaxf(bx1, __shfl_xor(bx1, o));
+                by1 = fmaxf(by1, __shfl_xor(by1, o)); bz1 = fmaxf(bz1, __shfl_xor(bz1, o));
+            }
+            if (lane == 0) {
+                s_box[wave][0] = bx0; s_box[wave][1] = by0; s_box[wave][2] = bz0;
+                s_box[wave][3] = bx1; s_box[wave][4] = by1; s_box[wave][5] = bz1;
+            }
+            __syncthreads();
+            const int noct = min(prm.disp_octaves, kCachedOctaves);
+            if (tid < noct) {
+                float m[6];
+                for (int k = 0; k < 6; ++k) m[k] = s_box[0][k];
+                for (int w = 1; w < Q / 64; ++w)
+                    for (int k = 0; k < 3; ++k) {
+                        m[k] = fminf(m[k], s_box[w][k]);
+                        m[k + 3] = fmaxf(m[k + 3], s_box[w][k + 3]);
+                    }
+                float f = prm.disp_frequency;
+                for (int k = 0; k < tid; ++k) f = f * 2.0f;
+                int geo[6] = { 0, 0, 0, 0, 0, 0 };
+                bool ok = true;
+                for (int k = 0; k < 3; ++k) {
+                    const float lo = floorf(m[k] * f), hi = floorf(m[k + 3] * f);
+                    ok = ok && (lo >= -1.0e6f) && (hi <= 1.0e6f) && (hi >= lo);
+                    if (ok) { geo[k] = (int)lo; geo[k + 3] = (int)hi - (int)lo + 2; }
+                }
+                if (!ok || geo[3] > kCacheEntries || geo[4] > kCacheEntries || geo[5] > kCacheEntries ||
+                    geo[3] * geo[4] * geo[5] > kCacheEntries)
+                    geo[3] = geo[4] = geo[5] = 0;
+                for (int k = 0; k < 6; ++k) s_geo[tid][k] = geo[k];
+                if (geo[3] == 0) ++uncached;
+            }
+            __syncthreads();
+            for (int k = 0; k < noct; ++k) {
+                const int nx = s_geo[k][3], ny = s_geo[k][4], nzc = s_geo[k][5];
+                const int cnt = nx * ny * nzc;
+                for (int i = tid; i < cnt; i += Q) {
+                    const int x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+                    s_lat[k][i] = lattice(s_geo[k][0] + x, s_geo[k][1] + y, s_geo[k][2] + z,
+                                          prm.disp_seed + (uint32_t)k);
+                }
+            }
+            __syncthreads();
+            for (int g = tid; g < G; g += Q) {
+                const float4 p4 = s_pos[g];
+                const float4 n4 = s_nrm[g];
+                float acc = 0.0f, amp = 1.0f, norm = 0.0f, freq = prm.disp_frequency;
+                for (int k = 0; k < prm.disp_octaves; ++k) {
+                    float nv;
+                    if (k < kCachedOctaves) {
+                        LatticeCache L;
+                        L.v = s_lat[k];
+                        L.lx = s_geo[k][0]; L.ly = s_geo[k][1]; L.lz = s_geo[k][2];
+                        L.nx = s_geo[k][3]; L.ny = s_geo[k][4]; L.nz = s_geo[k][5];
+                        nv = vnoise_cached(p4.x * freq, p4.y * freq, p4.z * freq, prm.disp_seed + (uint32_t)k, L);
+                    } else {
+                        nv = vnoise(p4.x * freq, p4.y * freq, p4.z * freq, prm.disp_seed + (uint32_t)k);
+                    }
+                    acc = acc + amp * nv;
+                    norm = norm + amp;
+                    amp = amp * 0.5f;
+                    freq = freq * 2.0f;
+                }
+                const float d = ob.disp_amplitude * (acc / norm);
+                float P[4] = { p4.x + d * n4.x, p4.y + d * n4.y, p4.z + d * n4.z, p4.w };
+                float e[4], q[4];
+                mat4_mul(ob.mv, P, e);
+                mat4_mul(vw.proj, e, q);
+                s_pos[g] = make_float4(e[0], e[1], e[2], e[3]);
+                const int32_t px = f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2));
+                const int32_t py = f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2));
+                s_pxl[g] = make_int2(px, py);
+                s_dep[g] = q[2] / q[3];
+            }
         }
         __syncthreads();
 
@@ -670,47 +935,65 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
         __syncthreads();
     }
     if (slow) atomicAdd(&fc->slow_blocks, slow);
+    if (uncached) atomicAdd(&fc->uncached_octaves, uncached);
     natom = (uint32_t)wave_sum_i((int)natom);
     if (lane == 0 && natom) atomicAdd(&fc->atomics, (unsigned long long)natom);
 }
 
-template <int N>
+template <int N, int MODE>
 static int raster_occupancy()
 {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_raster<N>, N * N, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_raster<N, MODE>, N * N, 0) != hipSuccess)
         blocks = 1;
     return blocks < 1 ? 1 : blocks;
 }
 
-int raster_grid(int patch_size, int num_cus)
+static int raster_mode(const DevParams& prm)
 {
+    return prm.displacement ? (prm.noise_cache ? 2 : 1) : 0;
+}
+
+int raster_grid(int patch_size, int num_cus, const DevParams& prm)
+{
+    const int mode = raster_mode(prm);
     int per_cu = 1;
+#define MP_OCC(NN)                                                                 \
+    per_cu = mode == 0 ? raster_occupancy<NN, 0>() : (mode == 1 ? raster_occupancy<NN, 1>() \
+                                                                : raster_occupancy<NN, 2>());
     switch (patch_size) {
-    case 8: per_cu = raster_occupancy<8>(); break;
-    case 16: per_cu = raster_occupancy<16>(); break;
-    case 32: per_cu = raster_occupancy<32>(); break;
+    case 8: MP_OCC(8) break;
+    case 16: MP_OCC(16) break;
+    case 32: MP_OCC(32) break;
     default: break;
     }
+#undef MP_OCC
     return per_cu * num_cus;
+}
+
+template <int N>
+static void launch_raster_n(int mode, int grid, hipStream_t s, const RangeRec* canon,
+                            const DevObject* objs, const DevView* views, const DevParams& prm,
+                            FrameCounters* fc, float* quad_store)
+{
+    if (mode == 0)
+        hipLaunchKernelGGL((k_raster<N, 0>), dim3(grid), dim3(N * N), 0, s, canon, objs, views, prm, fc, quad_store);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_raster<N, 1>), dim3(grid), dim3(N * N), 0, s, canon, objs, views, prm, fc, quad_store);
+    else
+        hipLaunchKernelGGL((k_raster<N, 2>), dim3(grid), dim3(N * N), 0, s, canon, objs, views, prm, fc, quad_store);
 }
 
 hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject* objs,
                          const DevView* views, const DevParams& prm, FrameCounters* fc,
                          float* quad_store, int grid, hipStream_t s)
 {
+    const int mode = raster_mode(prm);
     switch (patch_size) {
-    case 8:
-        hipLaunchKernelGGL(k_raster<8>, dim3(grid), dim3(64), 0, s, canon, objs, views, prm, fc, quad_store);
-        break;
-    case 16:
-        hipLaunchKernelGGL(k_raster<16>, dim3(grid), dim3(256), 0, s, canon, objs, views, prm, fc, quad_store);
-        break;
-    case 32:
-        hipLaunchKernelGGL(k_raster<32>, dim3(grid), dim3(1024), 0, s, canon, objs, views, prm, fc, quad_store);
-        break;
-    default:
-        return hipErrorInvalidValue;
+    case 8: launch_raster_n<8>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
+    case 16: launch_raster_n<16>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
+    case 32: launch_raster_n<32>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

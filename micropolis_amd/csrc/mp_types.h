@@ -69,6 +69,7 @@ struct DevParams {
     float disp_frequency;
     uint32_t disp_seed;
     int32_t disp_octaves;
+    int32_t noise_cache;         // cache lattice values per range in LDS
 };
 
 // Device-side frame counters, copied back once per frame.
@@ -81,6 +82,8 @@ struct FrameCounters {
     uint32_t total_ranges;       // canonical ranges (written by the scan)
     uint32_t pad_;
     unsigned long long atomics;  // (micropolygon, pixel) coverage events = key atomics
+    uint32_t raster_next;        // next canonical range to rasterise
+    uint32_t uncached_octaves;   // noise octaves evaluated by hashing
 };
 
 }  // namespace mp

@@ -203,7 +203,8 @@ def c3_batch(n_scenes=64, seed=3, width=640, height=480):
         n = int(np.floor(np.exp(rng.uniform(np.log(100), np.log(20000)))))
         radius = 2.0 * n ** (1.0 / 3.0)
         pats = random_patches(rng, n, radius)
-        dist = radius * np.sqrt(3.0) + 1.5 + rng.uniform(0.0, 3.0) * radius
+        # camera just outside the cloud: the nearest patches fill the screen
+        dist = radius * np.sqrt(3.0) + 1.0 + rng.uniform(0.0, 3.0)
         ang = rng.uniform(0, 2 * np.pi)
         eye = (dist * np.cos(ang), 0.3 * dist, dist * np.sin(ang))
         mv = look_at(eye, (0, 0, 0))
