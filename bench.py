@@ -11,6 +11,8 @@ Prints one JSON line.
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -58,6 +60,56 @@ def measure_batch(steps, use_graph):
             "same_micropolygons": int(mpolys) == int(st["micropolygons"])}
 
 
+def run_adjoint_child(steps):
+    """The adjoint measurement uses torch buffers; it runs in a child process
+    that initialises torch first."""
+    import subprocess
+    res = subprocess.run([sys.executable, os.path.abspath(__file__), "--adjoint-only", str(steps)],
+                         capture_output=True, text=True, timeout=600)
+    sys.stderr.write(res.stderr[-2000:])
+    if res.returncode != 0:
+        return {"error": "adjoint child exited with %d" % res.returncode}
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def measure_adjoint(steps):
+    """32 camera views of the displaced torus at 960x540 in one batch: forward
+    frame vs forward + hand-written adjoint (gradients w.r.t. all control
+    points, amplitude and albedo; visibility fixed; grids recomputed)."""
+    import torch
+    from micropolis_amd import Renderer, scenes
+
+    w = scenes.c5_views()
+    r = Renderer(**dict(w.options, write_rgba32=1))
+    base = w.views[0].objects[0]
+    r.load_patches(base.handle, base.patches, base.patch_type)
+    r.batch_begin([v.projection for v in w.views])
+    for vi, v in enumerate(w.views):
+        r.batch_draw(vi, base.handle, v.objects[0].matrix, base.color)
+    r.batch_finish()
+    r.replay(1, use_graph=True)
+    fwd_ms = r.replay(steps, use_graph=True)
+    V = len(w.views)
+    H, W = w.views[0].projection.viewport[1], w.views[0].projection.viewport[0]
+    dl = torch.randn((V, H, W, 4), device="cuda") * 0.1
+    g_cps = torch.empty((V,) + base.patches.shape, device="cuda")
+    g_col = torch.empty((V, 4), device="cuda")
+    g_amp = torch.empty((V,), device="cuda")
+    torch.cuda.synchronize()
+    bwd = []
+    for _ in range(steps + 1):
+        r.backward(dl.data_ptr(), g_cps.data_ptr(), g_col.data_ptr(), g_amp.data_ptr())
+        bwd.append(r.stats()["backward_ms"])
+    bwd_ms = float(np.median(bwd[1:]))
+    r.close()
+    print("adjoint: forward %.4f ms, backward %.4f ms for %d views (backward runs %s)" % (fwd_ms, bwd_ms, V, bwd),
+          file=sys.stderr, flush=True)
+    return {"workload": w.name, "views": V, "forward_ms": round(fwd_ms, 4), "backward_ms": round(bwd_ms, 4),
+            "forward_ms_per_view": round(fwd_ms / V, 4),
+            "fwd_plus_adjoint_ms_per_view": round((fwd_ms + bwd_ms) / V, 4),
+            "ratio_vs_forward": round((fwd_ms + bwd_ms) / fwd_ms, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +118,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
+    ap.add_argument("--adjoint-only", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -78,6 +132,12 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 4,
                           "bytes_written": b.numel() * 4, "launches": 3}))
+        return
+    if args.adjoint_only:
+        # torch (the gradient buffers) must initialise HIP before the library
+        import torch
+        torch.cuda.init()
+        print(json.dumps(measure_adjoint(args.adjoint_only)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only; this build has no multi-GPU path "
@@ -122,6 +182,7 @@ def main():
     achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
 
     batch = None if args.no_batch else measure_batch(max(3, min(args.steps, 10)), use_graph)
+    adjoint = None if args.no_adjoint else run_adjoint_child(max(3, min(args.steps, 10)))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -163,6 +224,7 @@ def main():
                      "kernel": "k_raster<16>", "alg_bytes": int(alg_bytes)},
         "cpu_baseline": cpu,
         "batch": batch,
+        "adjoint": adjoint,
     }
     r.close()
     print(json.dumps(line), flush=True)

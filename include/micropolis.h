@@ -94,6 +94,7 @@ typedef struct mp_stats {
     int64_t slow_blocks;              /* raster blocks on the wide-int path    */
     int64_t overflow_relaunches;      /* persistent split relaunches           */
     int64_t coverage_samples;         /* (micropolygon, pixel) samples written */
+    double  backward_ms;              /* device time of the last mp_backward    */
     double  frame_ms;                 /* mp_prepare..mp_finish device time     */
 } mp_stats;
 
@@ -196,6 +197,28 @@ int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* ms_per_fram
  * stages on the library's stream; stage_ms[0..3] receive the mean device ms
  * of split, canonical ordering, raster (dice+shade+sample) and resolve. */
 int mp_replay_stages(mp_context* ctx, int32_t n, double stage_ms[4]);
+
+/* ---- differentiable extension (no reference counterpart) ----
+ * Reverse pass of dice + displacement + flat shade + z-resolve for the last
+ * finished frame or batch, with visibility held fixed to its visible-id
+ * buffers (silhouette terms ignored) and every grid recomputed per range.
+ * All pointers are device pointers (HIP/torch allocations on this device).
+ *   dl_dcolor      : dL/dC for every view, concatenated in view order, each
+ *                    height*width*4 floats in the rgba32 layout (alpha unused)
+ *   grad_cps       : per object in draw order, n_patches*16*3 floats
+ *   grad_color     : per object 4 floats (albedo gradient, alpha 0)
+ *   grad_amplitude : per object 1 float
+ * Outputs are overwritten.  Bezier objects only.  Blocks until done. */
+int mp_backward(mp_context* ctx, const float* dl_dcolor, float* grad_cps, float* grad_color,
+                float* grad_amplitude);
+
+/* Device-to-device readback of a view's linear RGBA32F colour and ids
+ * (either may be NULL; needs write_rgba32 for the colour). */
+int mp_batch_read_device(mp_context* ctx, int32_t view, float* rgba32, uint32_t* ids);
+
+/* Replace a loaded patch set's control points from device memory (xyz
+ * floats, same count as loaded). */
+int mp_update_patches_device(mp_context* ctx, uint64_t handle, const float* xyz, size_t n_points);
 
 #ifdef __cplusplus
 }

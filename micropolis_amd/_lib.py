@@ -56,7 +56,7 @@ class mp_stats(C.Structure):
         ("micropolygons", C.c_int64), ("pass_count", C.c_int64),
         ("max_patches", C.c_int64), ("slow_blocks", C.c_int64),
         ("overflow_relaunches", C.c_int64), ("coverage_samples", C.c_int64),
-        ("frame_ms", C.c_double),
+        ("backward_ms", C.c_double), ("frame_ms", C.c_double),
     ]
 
 
@@ -92,6 +92,9 @@ EXPORTS = [
     ("mp_batch_read", C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
     ("mp_replay", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     ("mp_replay_stages", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    ("mp_backward", C.c_int, [_P, _P, _P, _P, _P]),
+    ("mp_batch_read_device", C.c_int, [_P, C.c_int32, _P, _P]),
+    ("mp_update_patches_device", C.c_int, [_P, C.c_uint64, _P, C.c_size_t]),
 ]
 
 _lib = None

@@ -120,9 +120,19 @@ struct mp_context {
     bool frame_valid = false;
     bool device_driven = false;  // the last frame can be replayed without host syncs
     hipGraphExec_t graph = nullptr;
-    bool work_sharing = false;   // spill queue between split groups (off: measured slower)
+    bool work_sharing = true;    // spill queue between split groups
+    uint32_t grad_floats = 0;
+    uint32_t* d_visible = nullptr; size_t visible_cap = 0;
+    uint32_t* d_next = nullptr;
+    int adjoint_grid_size = 0;
+    // level-synchronous split engine
+    int split_engine = 0;          // 0 persistent LDS-stack engine, 1 level-synchronous
+    RangeRec* d_level_a = nullptr; RangeRec* d_level_b = nullptr; uint32_t level_cap = 0;
+    uint32_t* d_level_cnt = nullptr; uint32_t* h_level_cnt = nullptr;
+    int level_launches = 0;        // levels launched before the persistent tail
+    int level_grid_size = 0;
     int spill_at = 192;
-    int linger = 0;
+    int linger = -1;             // groups that wait for spilled work (-1: one per CU)
 };
 
 #define HIPCHK(ctx, expr)                                                                  \
@@ -330,6 +340,8 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     if (const char* e = getenv("MP_SPILL_AT")) ctx->spill_at = atoi(e);
     if (const char* e = getenv("MP_LINGER")) ctx->linger = atoi(e);
     if (const char* e = getenv("MP_STACK_CAP")) ctx->stack_cap = std::max(256, atoi(e));
+    if (const char* e = getenv("MP_SPLIT_ENGINE")) ctx->split_engine = atoi(e);
+    ctx->level_launches = cfg->max_split_depth;
     if (const char* e = getenv("MP_RASTER_PER_CU")) ctx->raster_grid_size = ctx->num_cus * std::max(1, atoi(e));
     memset(&ctx->stats, 0, sizeof(ctx->stats));
     return ctx;
@@ -355,6 +367,8 @@ extern "C" void mp_destroy(mp_context* ctx)
     hipFree(ctx->d_queue); hipFree(ctx->d_queue_cnt); hipFree(ctx->d_shared_q); hipFree(ctx->d_shared_cnt);
     hipFree(ctx->d_chunks); hipFree(ctx->d_chunk_count); hipFree(ctx->d_chunk_ready); hipFree(ctx->d_share);
     hipFree(ctx->d_keys);
+    hipFree(ctx->d_visible); hipFree(ctx->d_next);
+    hipFree(ctx->d_level_a); hipFree(ctx->d_level_b); hipFree(ctx->d_level_cnt); hipHostFree(ctx->h_level_cnt);
     hipFree(ctx->d_seg_a); hipFree(ctx->d_seg_b); hipFree(ctx->d_flags); hipFree(ctx->d_blk);
     hipFree(ctx->d_draw); hipFree(ctx->d_bucket); hipFree(ctx->d_canon);
     hipFree(ctx->d_counts); hipFree(ctx->d_offsets); hipFree(ctx->d_cursor);
@@ -613,6 +627,7 @@ static int upload_tables(mp_context* ctx)
             d.n_patches = ps.n_patches;
             d.view = v;
             d.gpid_base = g;
+            d.draw_index = order[oi];
             g += (uint32_t)ps.n_patches;
             ++oi;
         }
@@ -620,6 +635,17 @@ static int upload_tables(mp_context* ctx)
     }
     ctx->n_gpids = g;
     ctx->total_roots = g;
+    // control-point gradient offsets in draw-call order
+    {
+        std::vector<uint32_t> off(no + 1, 0);
+        for (int i = 0; i < no; ++i) {
+            const PatchSet& ps = ctx->patches[ctx->objects[i].handle];
+            off[i + 1] = off[i] + (uint32_t)ps.n_patches * (ps.type == MP_GREGORY ? 20u : 16u) * 3u;
+        }
+        for (int i = 0; i < no; ++i) ctx->h_objs[i].grad_offset = off[ctx->h_objs[i].draw_index];
+        ctx->grad_floats = off[no];
+    }
+    uint32_t pix = 0;
     for (int v = 0; v < nv; ++v) {
         DevView& dv = ctx->h_views[v];
         make_view(ctx->view_projs[v], ctx->cfg.clear_color, ctx->cfg.subpixel_bits, dv);
@@ -631,6 +657,8 @@ static int upload_tables(mp_context* ctx)
         dv.rgba32 = vb.rgba32;
         dv.first_gpid = first[v];
         dv.gpid_end = end[v];
+        dv.pixel_offset = pix;
+        pix += (uint32_t)dv.width * (uint32_t)dv.height;
     }
     if (nv) HIPCHK(ctx, hipMemcpyAsync(ctx->d_views, ctx->h_views, nv * sizeof(DevView), hipMemcpyHostToDevice, ctx->stream));
     if (no) HIPCHK(ctx, hipMemcpyAsync(ctx->d_objs, ctx->h_objs, no * sizeof(DevObject), hipMemcpyHostToDevice, ctx->stream));
@@ -640,15 +668,30 @@ static int upload_tables(mp_context* ctx)
 
 static int ensure_queues(mp_context* ctx)
 {
+    int rc;
     const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
     int groups = ctx->cfg.local_bns_work_groups;
     if (groups <= 0) {
         groups = ctx->num_cus * split_persistent_occupancy((size_t)ctx->stack_cap * sizeof(RangeRec));
     }
     ctx->split_groups = groups;
-    const size_t stride = balanced ? 1 : (ctx->total_roots + groups - 1) / groups;
+    if (ctx->split_engine == 1) {
+        // level lists: sized for the widest level; overflow re-renders bigger
+        const uint32_t want = std::max<uint32_t>(ctx->level_cap, std::max<uint32_t>(65536, 8 * ctx->total_roots));
+        if (want > ctx->level_cap || !ctx->d_level_a) {
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            if ((rc = dev_alloc(ctx, &ctx->d_level_a, want)) || (rc = dev_alloc(ctx, &ctx->d_level_b, want))) return rc;
+            ctx->level_cap = want;
+            invalidate_graph(ctx);
+        }
+        if (!ctx->d_level_cnt) {
+            if ((rc = dev_alloc(ctx, &ctx->d_level_cnt, 64))) return rc;
+            HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_level_cnt, 64 * sizeof(uint32_t), 0));
+        }
+    }
+    const size_t roots_or_left = std::max<size_t>(ctx->total_roots, ctx->split_engine == 1 ? ctx->level_cap : 0);
+    const size_t stride = balanced && ctx->split_engine != 1 ? 1 : (roots_or_left + groups - 1) / groups;
     const size_t entries = std::max<size_t>(1, stride * groups);
-    int rc;
     if (entries > ctx->queue_entries) {
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         if ((rc = dev_alloc(ctx, &ctx->d_queue, entries))) return rc;
@@ -694,7 +737,8 @@ static SplitQueues split_queues(mp_context* ctx)
     const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
     q.queue = ctx->d_queue;
     q.queue_cnt = ctx->d_queue_cnt;
-    q.stride = balanced ? 0 : (int64_t)((ctx->total_roots + ctx->split_groups - 1) / ctx->split_groups);
+    const size_t left = std::max<size_t>(ctx->total_roots, ctx->split_engine == 1 ? ctx->level_cap : 0);
+    q.stride = (balanced && ctx->split_engine != 1) ? 0 : (int64_t)((left + ctx->split_groups - 1) / ctx->split_groups);
     q.shared_q = ctx->d_shared_q;
     q.shared_cnt = ctx->d_shared_cnt;
     q.n_groups = ctx->split_groups;
@@ -704,7 +748,7 @@ static SplitQueues split_queues(mp_context* ctx)
     q.chunk_ready = ctx->d_chunk_ready;
     q.chunk_cap = ctx->work_sharing ? ctx->chunk_cap : 0u;
     q.spill_at = ctx->spill_at;
-    q.linger = ctx->linger;
+    q.linger = ctx->linger >= 0 ? ctx->linger : ctx->num_cus;
     q.share = ctx->d_share;
     return q;
 }
@@ -736,7 +780,13 @@ static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
     HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
     SplitQueues q = split_queues(ctx);
-    if (ctx->total_roots) {
+    if (ctx->total_roots && ctx->split_engine == 1) {
+        if (!ctx->level_grid_size) ctx->level_grid_size = level_grid(ctx->num_cus);
+        HIPCHK(ctx, launch_split_levels(ctx->d_objs, (int)ctx->objects.size(), ctx->d_views, ctx->prm,
+                                        ctx->total_roots, ctx->d_level_a, ctx->d_level_b, ctx->level_cap,
+                                        ctx->d_level_cnt, ctx->level_launches, q, canon_buffers(ctx), ctx->d_fc,
+                                        ctx->level_grid_size, ctx->stream));
+    } else if (ctx->total_roots) {
         HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
                                        balanced, ctx->stream));
         HIPCHK(ctx, launch_split_persistent(ctx->d_objs, ctx->d_views, ctx->prm, q,
@@ -849,7 +899,15 @@ static int execute_frame(mp_context* ctx)
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
             const uint32_t ovf = ctx->h_fc->overflow;
             if (ovf & 4u) return fail(ctx, MP_E_HIP, "split work sharing timed out");
-            if (!(ovf & 3u)) break;
+            if (!(ovf & 11u)) break;
+            if (ovf & 8u) {
+                ctx->level_cap *= 4;   // a split level outgrew its list
+                HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+                if ((rc = dev_alloc(ctx, &ctx->d_level_a, ctx->level_cap)) ||
+                    (rc = dev_alloc(ctx, &ctx->d_level_b, ctx->level_cap)))
+                    return rc;
+                if ((rc = ensure_queues(ctx))) return rc;
+            }
             ++relaunches;
             if (ovf & 2u) {
                 const int full = 64 * (ctx->cfg.max_split_depth + 2);
@@ -870,6 +928,17 @@ static int execute_frame(mp_context* ctx)
             left -= ppp;
         } while (left > 0);
         max_patches = ctx->h_fc->max_stack;
+        if (ctx->split_engine == 1) {
+            // launch only as many levels as this scene needs (+1); the
+            // persistent tail still finishes anything deeper
+            HIPCHK(ctx, hipMemcpy(ctx->h_level_cnt, ctx->d_level_cnt, 64 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            int deepest = 0;
+            for (int L = 0; L <= ctx->level_launches && L < 63; ++L) {
+                if (ctx->h_level_cnt[L]) deepest = L;
+                max_patches = std::max<int64_t>(max_patches, ctx->h_level_cnt[L]);
+            }
+            ctx->level_launches = std::min(ctx->cfg.max_split_depth, deepest + 2);
+        }
     } else {
         HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
         if ((rc = run_segment_split(ctx, draw_total, dropped, max_patches))) return rc;
@@ -1087,5 +1156,70 @@ extern "C" int mp_replay_stages(mp_context* ctx, int32_t n, double stage_ms[4])
         if (e) hipEventDestroy(e);
     if (rc) return rc;
     for (int k = 0; k < 4; ++k) stage_ms[k] = acc[k] / n;
+    return MP_OK;
+}
+
+extern "C" int mp_backward(mp_context* ctx, const float* dl_dcolor, float* grad_cps, float* grad_color,
+                           float* grad_amplitude)
+{
+    if (!ctx) return MP_E_INVALID;
+    if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame");
+    if (!dl_dcolor || !grad_cps || !grad_color || !grad_amplitude) return fail(ctx, MP_E_INVALID, "null pointer");
+    if (ctx->cfg.reyes_patch_size != 8 && ctx->cfg.reyes_patch_size != 16)
+        return fail(ctx, MP_E_INVALID, "mp_backward supports reyes_patch_size 8 and 16");
+    for (const ObjectReq& o : ctx->objects)
+        if (ctx->patches[o.handle].type != MP_BEZIER) return fail(ctx, MP_E_INVALID, "mp_backward needs Bezier patches");
+    int rc;
+    if (ctx->list_cap > ctx->visible_cap) {
+        if ((rc = dev_alloc(ctx, &ctx->d_visible, ctx->list_cap))) return rc;
+        ctx->visible_cap = ctx->list_cap;
+    }
+    if (!ctx->d_next && (rc = dev_alloc(ctx, &ctx->d_next, 1))) return rc;
+    if (!ctx->adjoint_grid_size) ctx->adjoint_grid_size = adjoint_grid(ctx->cfg.reyes_patch_size, ctx->num_cus);
+    const int no = (int)ctx->objects.size();
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(ctx, launch_fill_u32(ctx->d_visible, ctx->visible_cap, 0u, ctx->stream));
+    HIPCHK(ctx, launch_fill_u32(ctx->d_next, 1, 0u, ctx->stream));
+    HIPCHK(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(grad_cps), ctx->grad_floats, 0u, ctx->stream));
+    HIPCHK(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(grad_color), (size_t)no * 4, 0u, ctx->stream));
+    HIPCHK(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(grad_amplitude), (size_t)no, 0u, ctx->stream));
+    HIPCHK(ctx, launch_adjoint(ctx->cfg.reyes_patch_size, ctx->d_canon, ctx->d_objs, ctx->d_views, ctx->n_views,
+                               ctx->max_pixels, ctx->prm, ctx->d_fc, ctx->d_quad, dl_dcolor, ctx->d_visible,
+                               ctx->d_next, grad_cps, grad_color, grad_amplitude, ctx->adjoint_grid_size,
+                               ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    ctx->stats.backward_ms = ms;
+    return MP_OK;
+}
+
+extern "C" int mp_batch_read_device(mp_context* ctx, int32_t view, float* rgba32, uint32_t* ids)
+{
+    if (!ctx) return MP_E_INVALID;
+    if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame");
+    if (view < 0 || view >= ctx->n_views) return fail(ctx, MP_E_INVALID, "bad view index");
+    const ViewBuffers& vb = ctx->views[view];
+    const size_t np = (size_t)vb.width * vb.height;
+    if (rgba32) {
+        if (!vb.rgba32) return fail(ctx, MP_E_STATE, "write_rgba32 is off");
+        HIPCHK(ctx, hipMemcpyAsync(rgba32, vb.rgba32, np * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (ids) HIPCHK(ctx, hipMemcpyAsync(ids, vb.ids, np * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MP_OK;
+}
+
+extern "C" int mp_update_patches_device(mp_context* ctx, uint64_t handle, const float* xyz, size_t n_points)
+{
+    if (!ctx) return MP_E_INVALID;
+    auto it = ctx->patches.find(handle);
+    if (it == ctx->patches.end()) return fail(ctx, MP_E_NOTFOUND, "unknown patch handle");
+    const size_t per = it->second.type == MP_GREGORY ? 20 : 16;
+    if (n_points != per * (size_t)it->second.n_patches) return fail(ctx, MP_E_INVALID, "point count differs from the loaded set");
+    if (n_points && !xyz) return fail(ctx, MP_E_INVALID, "null patch data");
+    HIPCHK(ctx, launch_xyz_to_cps(xyz, it->second.d_cps, n_points, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MP_OK;
 }

@@ -79,6 +79,20 @@ hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           int patch_size, int write_rgba32, hipStream_t s);
 
 hipError_t launch_init_roots(int obj, uint32_t n, RangeRec* out, hipStream_t s);
+
+// reverse pass of the differentiable extension
+hipError_t launch_adjoint(int patch_size, const RangeRec* canon, const DevObject* objs,
+                          const DevView* views, int n_views, int max_pixels, const DevParams& prm,
+                          const FrameCounters* fc, const float* quad_store, const float* dl_dcolor,
+                          uint32_t* visible, uint32_t* next, float* grad_cps, float* grad_color,
+                          float* grad_amp, int grid, hipStream_t s);
+int adjoint_grid(int patch_size, int num_cus);
+int level_grid(int num_cus);
+hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView* views, const DevParams& prm,
+                               uint32_t total_roots, RangeRec* list_a, RangeRec* list_b, uint32_t list_cap,
+                               uint32_t* level_cnt, int n_levels, const SplitQueues& q,
+                               const CanonBuffers& cb, FrameCounters* fc, int grid, hipStream_t s);
+hipError_t launch_xyz_to_cps(const float* xyz, float4* cps, size_t n, hipStream_t s);
 int split_persistent_occupancy(size_t lds_bytes);
 
 }  // namespace mp

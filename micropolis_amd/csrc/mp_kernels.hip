@@ -240,7 +240,9 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
         }
         sh -= scnt;
         if (cnt == 0 && scnt == 0) {
-            if (!sharing || !q.linger) {
+            // only the first `linger` groups wait for spilled work: a bounded
+            // set of slow pollers cannot flood the shared state lines
+            if (!sharing || wid >= q.linger) {
                 if (sharing && active && lane == 0) atomicSub(&q.share->active, 1u);
                 active = false;
                 break;
@@ -261,9 +263,7 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
                     } else if (__hip_atomic_load(&q.share->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
                         break;
                     }
-                    // back off so that idle groups do not flood the shared lines
-                    if (it < 8) __builtin_amdgcn_s_sleep(8);
-                    else __builtin_amdgcn_s_sleep(64);
+                    __builtin_amdgcn_s_sleep(127);
                 }
             }
             c = __shfl(c, 0);
@@ -713,10 +713,18 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     uint32_t slow = 0, natom = 0, uncached = 0;
     __shared__ uint32_t s_next;
 
+    // chunked dynamic scheduling: one atomic per `chunk` ranges keeps the
+    // shared counter off the critical path for million-range batches
+    const uint32_t chunk = max(1u, min(64u, total / (gridDim.x * 16u)));
+    uint32_t r = 0, r_end = 0;
     for (;;) {
-        if (tid == 0) s_next = atomicAdd(&fc->raster_next, 1u);
-        __syncthreads();
-        const uint32_t r = s_next;
+        if (r >= r_end) {
+            if (tid == 0) s_next = atomicAdd(&fc->raster_next, chunk);
+            __syncthreads();
+            r = s_next;
+            r_end = min(r + chunk, total);
+            __syncthreads();
+        }
         if (r >= total) break;
         const RangeRec rec = canon[r];
         const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
@@ -932,6 +940,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
                 }
             }
         }
+        ++r;
         __syncthreads();
     }
     if (slow) atomicAdd(&fc->slow_blocks, slow);
@@ -1090,6 +1099,475 @@ int split_persistent_occupancy(size_t lds_bytes)
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_split_persistent, 64, lds_bytes) != hipSuccess)
         blocks = 1;
     return blocks < 1 ? 1 : blocks;
+}
+
+}  // namespace mp
+
+// ---------------------------------------------------------------------------
+// Reverse pass of the differentiable extension.  Loss gradient dL/dC per
+// pixel (linear RGB) -> control points, displacement amplitude and albedo,
+// with visibility held fixed to the forward visible-id buffer and the grids
+// recomputed per range (no stored forward state):
+//   C_c = (0.015 + s) * albedo_c,  s = max(n.l, 0),  n = unit(cross(dv, du))
+//   corner e = MV * (P + amp * noise(P) * unit(cross(dP/dv, dP/du)))
+
+namespace mp {
+
+__global__ void k_xyz_to_cps(const float* __restrict__ xyz, float4* __restrict__ cps, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cps[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 1.0f);
+}
+
+hipError_t launch_xyz_to_cps(const float* xyz, float4* cps, size_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xyz_to_cps, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, xyz, cps, n);
+    return hipGetLastError();
+}
+
+// flag every range that owns a visible micropolygon
+__global__ __launch_bounds__(256) void k_adj_mark(const DevView* __restrict__ views, int Q,
+                                                  uint32_t* __restrict__ visible)
+{
+    const DevView& vw = views[blockIdx.y];
+    const uint32_t n = (uint32_t)vw.width * (uint32_t)vw.height;
+    for (uint32_t o = blockIdx.x * 256 + threadIdx.x; o < n; o += gridDim.x * 256) {
+        const uint32_t id = vw.ids[o];
+        if (id != 0xFFFFFFFFu) visible[vw.range_base + id / (uint32_t)Q] = 1u;
+    }
+}
+
+struct V3 { float x, y, z; };
+MP_DEV V3 mk(float x, float y, float z) { V3 r = { x, y, z }; return r; }
+MP_DEV V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+MP_DEV V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+MP_DEV V3 operator*(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+MP_DEV float vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+MP_DEV V3 vcross(V3 a, V3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+// noise value and gradient (same lattice and octave sum as noise3)
+MP_DEV float noise3_grad(const DevParams& prm, float x, float y, float z, V3& grad)
+{
+    float acc = 0.0f, amp = 1.0f, norm = 0.0f, freq = prm.disp_frequency;
+    V3 g = mk(0.f, 0.f, 0.f);
+    for (int k = 0; k < prm.disp_octaves; ++k) {
+        const uint32_t seed = prm.disp_seed + (uint32_t)k;
+        const float px = x * freq, py = y * freq, pz = z * freq;
+        const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+        const int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
+        const float tx = px - fx, ty = py - fy, tz = pz - fz;
+        const float wx = fade(tx), wy = fade(ty), wz = fade(tz);
+        const float c000 = lattice(ix, iy, iz, seed),         c100 = lattice(ix + 1, iy, iz, seed);
+        const float c010 = lattice(ix, iy + 1, iz, seed),     c110 = lattice(ix + 1, iy + 1, iz, seed);
+        const float c001 = lattice(ix, iy, iz + 1, seed),     c101 = lattice(ix + 1, iy, iz + 1, seed);
+        const float c011 = lattice(ix, iy + 1, iz + 1, seed), c111 = lattice(ix + 1, iy + 1, iz + 1, seed);
+        const float x00 = lerpf_(c000, c100, wx), x10 = lerpf_(c010, c110, wx);
+        const float x01 = lerpf_(c001, c101, wx), x11 = lerpf_(c011, c111, wx);
+        const float y0 = lerpf_(x00, x10, wy), y1 = lerpf_(x01, x11, wy);
+        acc = acc + amp * lerpf_(y0, y1, wz);
+        const float dwx = lerpf_(lerpf_(c100 - c000, c110 - c010, wy), lerpf_(c101 - c001, c111 - c011, wy), wz);
+        const float dwy = lerpf_(x10 - x00, x11 - x01, wz);
+        const float dwz = y1 - y0;
+        const float s = amp * freq;
+        g = g + mk(dwx * (6.0f * tx * (1.0f - tx)), dwy * (6.0f * ty * (1.0f - ty)),
+                   dwz * (6.0f * tz * (1.0f - tz))) * s;
+        norm = norm + amp;
+        amp = amp * 0.5f;
+        freq = freq * 2.0f;
+    }
+    grad = g * (1.0f / norm);
+    return acc / norm;
+}
+
+MP_DEV float wave_sum_f(float x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+template <int N>
+__global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ canon,
+                                                   const DevObject* __restrict__ objs,
+                                                   const DevView* __restrict__ views, DevParams prm,
+                                                   const FrameCounters* __restrict__ fc,
+                                                   const float* __restrict__ quad_store,
+                                                   const float* __restrict__ dl_dcolor,
+                                                   const uint32_t* __restrict__ visible, uint32_t* next,
+                                                   float* __restrict__ grad_cps, float* __restrict__ grad_color,
+                                                   float* __restrict__ grad_amp)
+{
+    constexpr int G1 = N + 1, G = G1 * G1, Q = N * N, W = Q / 64, PPT = (G + Q - 1) / Q;
+    __shared__ float4 s_cp[16];
+    __shared__ float4 s_e[G];
+    __shared__ int2 s_pxl[G];
+    __shared__ float s_ge[G][3];
+    __shared__ float s_red[W][52];
+    __shared__ uint32_t s_next, s_next_end;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t total = fc->total_ranges;
+    const int sh = prm.subpixel_bits;
+    const V3 L = [] { float lx = 4.0f, ly = 3.0f, lz = 8.0f; const float inv = 1.0f / sqrtf(89.0f);
+                      return mk(lx * inv, ly * inv, lz * inv); }();
+
+    const uint32_t chunk = max(1u, min(64u, total / (gridDim.x * 16u)));
+    uint32_t r = 0, r_end = 0;
+    for (;;) {
+        if (tid == 0) {
+            // next visible range: claim chunks of ranges, skip hidden ones
+            for (;;) {
+                while (r < r_end && !visible[r]) ++r;
+                if (r < r_end) break;
+                r = atomicAdd(next, chunk);
+                if (r >= total) break;
+                r_end = min(r + chunk, total);
+            }
+            s_next = r;
+            s_next_end = r_end;
+        }
+        __syncthreads();
+        r = s_next;
+        r_end = s_next_end;
+        if (r >= total) break;
+        const RangeRec rec = canon[r];
+        const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
+        const DevView& vw = views[ob.view];
+        if (tid < 16) s_cp[tid] = ob.cps[(size_t)rec.pid * 16 + tid];
+        __syncthreads();
+
+        // forward state of the grid points owned by this thread
+        V3 Pu[PPT], Pv[PPT], nh[PPT], gn[PPT];
+        float nlen[PPT], nzv[PPT], dd[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const int g = tid + k * Q;
+            if (g < G) {
+                const int nu = g % G1, nv = g / G1;
+                const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
+                const float uu = rec.umin + (rec.umax - rec.umin) * tu;
+                const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
+                float P[4];
+                eval_bezier(s_cp, uu, vv, P);
+                float su = 1.0f - uu, sv = 1.0f - vv, a[4], b[4], da[4], db[4];
+                bern_u(uu, su, a); bern_v(vv, sv, b); dbern(uu, su, da); dbern(vv, sv, db);
+                V3 pu = mk(0.f, 0.f, 0.f), pv = mk(0.f, 0.f, 0.f);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float4 C = s_cp[4 * i + j];
+                        pu = pu + mk(C.x, C.y, C.z) * (da[i] * b[j]);
+                        pv = pv + mk(C.x, C.y, C.z) * (a[i] * db[j]);
+                    }
+                const V3 Nn = vcross(pv, pu);
+                const float len = sqrtf(vdot(Nn, Nn));
+                const V3 h = len > 0.0f ? Nn * (1.0f / len) : mk(0.f, 0.f, 0.f);
+                V3 gz = mk(0.f, 0.f, 0.f);
+                float nzz = 0.0f, d = 0.0f;
+                if (prm.displacement) {
+                    nzz = noise3_grad(prm, P[0], P[1], P[2], gz);
+                    d = ob.disp_amplitude * nzz;
+                }
+                Pu[k] = pu; Pv[k] = pv; nh[k] = h; gn[k] = gz; nlen[k] = len; nzv[k] = nzz; dd[k] = d;
+                // forward displaced eye position and pixel position: P + d * unit
+                // normal with the same operations as the forward dice
+                float Pd[4] = { P[0], P[1], P[2], P[3] };
+                if (prm.displacement) { Pd[0] = P[0] + d * h.x; Pd[1] = P[1] + d * h.y; Pd[2] = P[2] + d * h.z; }
+                float e[4], q[4];
+                mat4_mul(ob.mv, Pd, e);
+                mat4_mul(vw.proj, e, q);
+                s_e[g] = make_float4(e[0], e[1], e[2], e[3]);
+                s_pxl[g] = make_int2(f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2)),
+                                     f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2)));
+                s_ge[g][0] = s_ge[g][1] = s_ge[g][2] = 0.0f;
+            }
+        }
+        __syncthreads();
+
+        // micropolygon: gather dL/dC from the pixels it is visible in
+        const int u = tid % N, v = tid / N, g0 = u + v * G1;
+        const uint32_t myid = (r - vw.range_base) * (uint32_t)Q + (uint32_t)tid;
+        const float s_fwd = quad_store[(size_t)r * Q + tid];
+        float gs = 0.0f, ga0 = 0.0f, ga1 = 0.0f, ga2 = 0.0f;
+        if (s_fwd == s_fwd) {
+            const int2 x0 = s_pxl[g0], x1 = s_pxl[g0 + 1], x2 = s_pxl[g0 + G1], x3 = s_pxl[g0 + G1 + 1];
+            const int X0 = max(0, min(min(x0.x, x1.x), min(x2.x, x3.x)) >> sh);
+            const int X1 = min(vw.width - 1, max(max(x0.x, x1.x), max(x2.x, x3.x)) >> sh);
+            const int Y0 = max(0, min(min(x0.y, x1.y), min(x2.y, x3.y)) >> sh);
+            const int Y1 = min(vw.height - 1, max(max(x0.y, x1.y), max(x2.y, x3.y)) >> sh);
+            {
+                for (int Y = Y0; Y <= Y1; ++Y)
+                    for (int X = X0; X <= X1; ++X) {
+                        const size_t o = (size_t)Y * vw.width + X;
+                        if (vw.ids[o] != myid) continue;
+                        const float* dl = dl_dcolor + ((size_t)vw.pixel_offset + o) * 4;
+                        gs += dl[0] * ob.color[0] + dl[1] * ob.color[1] + dl[2] * ob.color[2];
+                        ga0 += dl[0]; ga1 += dl[1]; ga2 += dl[2];
+                    }
+            }
+        }
+        const float acs = 0.015f + (s_fwd == s_fwd ? s_fwd : 0.0f);
+        ga0 *= acs; ga1 *= acs; ga2 *= acs;
+        if (gs != 0.0f) {
+            const float4 a0 = s_e[g0], a1 = s_e[g0 + 1], a2 = s_e[g0 + G1], a3 = s_e[g0 + G1 + 1];
+            const V3 p0 = mk(a0.x, a0.y, a0.z), p1 = mk(a1.x, a1.y, a1.z);
+            const V3 p2 = mk(a2.x, a2.y, a2.z), p3 = mk(a3.x, a3.y, a3.z);
+            const V3 du = ((p1 - p0) + (p3 - p2)) * 0.5f;
+            const V3 dv = ((p2 - p0) + (p3 - p1)) * 0.5f;
+            const V3 m = vcross(dv, du);
+            const float ml = sqrtf(vdot(m, m));
+            if (ml > 0.0f) {
+                const V3 n = m * (1.0f / ml);
+                if (vdot(n, L) > 0.0f) {
+                    const V3 gm = (L - n * vdot(n, L)) * (gs / ml);
+                    const V3 gdv = vcross(du, gm), gdu = vcross(gm, dv);
+                    const V3 c0 = (gdu + gdv) * -0.5f, c1 = (gdu - gdv) * 0.5f;
+                    const V3 c2 = (gdv - gdu) * 0.5f, c3 = (gdu + gdv) * 0.5f;
+                    const int gi[4] = { g0, g0 + 1, g0 + G1, g0 + G1 + 1 };
+                    const V3 cc[4] = { c0, c1, c2, c3 };
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        atomicAdd(&s_ge[gi[k]][0], cc[k].x);
+                        atomicAdd(&s_ge[gi[k]][1], cc[k].y);
+                        atomicAdd(&s_ge[gi[k]][2], cc[k].z);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // grid points: eye -> displaced -> surface -> control points
+        float gcp[48];
+#pragma unroll
+        for (int i = 0; i < 48; ++i) gcp[i] = 0.0f;
+        float gamp = 0.0f;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const int g = tid + k * Q;
+            if (g >= G) continue;
+            const V3 ge = mk(s_ge[g][0], s_ge[g][1], s_ge[g][2]);
+            if (ge.x == 0.0f && ge.y == 0.0f && ge.z == 0.0f) continue;
+            const float* M = ob.mv;
+            const V3 gPd = mk(M[0] * ge.x + M[1] * ge.y + M[2] * ge.z, M[4] * ge.x + M[5] * ge.y + M[6] * ge.z,
+                              M[8] * ge.x + M[9] * ge.y + M[10] * ge.z);
+            V3 gP = gPd, gPu = mk(0.f, 0.f, 0.f), gPv = mk(0.f, 0.f, 0.f);
+            if (prm.displacement) {
+                const float nd = vdot(nh[k], gPd);
+                gamp += nzv[k] * nd;
+                gP = gP + gn[k] * (ob.disp_amplitude * nd);
+                if (nlen[k] > 0.0f) {
+                    const V3 gnh = gPd * dd[k];
+                    const V3 gN = (gnh - nh[k] * vdot(nh[k], gnh)) * (1.0f / nlen[k]);
+                    gPv = vcross(Pu[k], gN);
+                    gPu = vcross(gN, Pv[k]);
+                }
+            }
+            const int nu = g % G1, nv = g / G1;
+            const float uu = rec.umin + (rec.umax - rec.umin) * ((float)nu / (float)N);
+            const float vv = rec.vmin + (rec.vmax - rec.vmin) * ((float)nv / (float)N);
+            float su = 1.0f - uu, sv = 1.0f - vv, a[4], b[4], da[4], db[4];
+            bern_u(uu, su, a); bern_v(vv, sv, b); dbern(uu, su, da); dbern(vv, sv, db);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int c = 4 * i + j;
+                    const float w = a[i] * b[j], wu = da[i] * b[j], wv = a[i] * db[j];
+                    gcp[3 * c + 0] += w * gP.x + wu * gPu.x + wv * gPv.x;
+                    gcp[3 * c + 1] += w * gP.y + wu * gPu.y + wv * gPv.y;
+                    gcp[3 * c + 2] += w * gP.z + wu * gPu.z + wv * gPv.z;
+                }
+        }
+        // reduce over the work-group: waves by shuffles, then across waves
+#pragma unroll
+        for (int i = 0; i < 48; ++i) {
+            const float t = wave_sum_f(gcp[i]);
+            if (lane == 0) s_red[wave][i] = t;
+        }
+        {
+            const float t0 = wave_sum_f(gamp), t1 = wave_sum_f(ga0), t2 = wave_sum_f(ga1), t3 = wave_sum_f(ga2);
+            if (lane == 0) { s_red[wave][48] = t0; s_red[wave][49] = t1; s_red[wave][50] = t2; s_red[wave][51] = t3; }
+        }
+        __syncthreads();
+        if (tid < 52) {
+            float t = 0.0f;
+            for (int w = 0; w < W; ++w) t += s_red[w][tid];
+            if (t != 0.0f) {
+                if (tid < 48) atomicAdd(&grad_cps[(size_t)ob.grad_offset + (size_t)rec.pid * 48 + tid], t);
+                else if (tid == 48) atomicAdd(&grad_amp[ob.draw_index], t);
+                else atomicAdd(&grad_color[ob.draw_index * 4 + (tid - 49)], t);
+            }
+        }
+        ++r;
+        __syncthreads();
+    }
+}
+
+int adjoint_grid(int patch_size, int num_cus)
+{
+    int blocks = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (patch_size == 8) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_adjoint<8>, 64, 0);
+    if (patch_size == 16) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_adjoint<16>, 256, 0);
+    if (e != hipSuccess || blocks < 1) blocks = 1;
+    return blocks * num_cus;
+}
+
+hipError_t launch_adjoint(int patch_size, const RangeRec* canon, const DevObject* objs,
+                          const DevView* views, int n_views, int max_pixels, const DevParams& prm,
+                          const FrameCounters* fc, const float* quad_store, const float* dl_dcolor,
+                          uint32_t* visible, uint32_t* next, float* grad_cps, float* grad_color,
+                          float* grad_amp, int grid, hipStream_t s)
+{
+    const int Q = patch_size * patch_size;
+    int bx = (max_pixels + 255) / 256;
+    bx = bx > 1024 ? 1024 : (bx < 1 ? 1 : bx);
+    hipLaunchKernelGGL(k_adj_mark, dim3(bx, n_views), dim3(256), 0, s, views, Q, visible);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (patch_size == 8)
+        hipLaunchKernelGGL(k_adjoint<8>, dim3(grid), dim3(64), 0, s, canon, objs, views, prm, fc, quad_store,
+                           dl_dcolor, visible, next, grad_cps, grad_color, grad_amp);
+    else if (patch_size == 16)
+        hipLaunchKernelGGL(k_adjoint<16>, dim3(grid), dim3(256), 0, s, canon, objs, views, prm, fc, quad_store,
+                           dl_dcolor, visible, next, grad_cps, grad_color, grad_amp);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace mp
+
+// ---------------------------------------------------------------------------
+// Level-synchronous split engine: every range of split level L (all objects,
+// all views) is bounded by one launch; split children are appended to level
+// L+1 and DRAW ranges to the frame's draw list with one atomic per wave and
+// target.  The DRAW set is the same as the persistent engine's; the frame's
+// canonical ordering makes the emission order irrelevant.
+
+namespace mp {
+
+__global__ void k_level_roots(const DevObject* __restrict__ objs, int n_objs, uint32_t total,
+                              RangeRec* out, uint32_t* level_cnt, int n_levels)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < (uint32_t)n_levels + 1) level_cnt[g] = (g == 0) ? total : 0u;
+    if (g >= total) return;
+    int lo = 0, hi = n_objs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (objs[mid].gpid_base <= g) lo = mid; else hi = mid - 1;
+    }
+    RangeRec r;
+    r.pid = g - objs[lo].gpid_base;
+    r.obj_depth = (uint32_t)lo;
+    r.umin = 0.0f; r.vmin = 0.0f; r.umax = 1.0f; r.vmax = 1.0f;
+    out[g] = r;
+}
+
+__global__ __launch_bounds__(256) void k_level(const DevObject* __restrict__ objs,
+                                               const DevView* __restrict__ views, DevParams prm,
+                                               const RangeRec* __restrict__ cur, RangeRec* __restrict__ nxt,
+                                               uint32_t* level_cnt, int L, uint32_t nxt_cap,
+                                               CanonBuffers cb, FrameCounters* fc)
+{
+    const uint32_t n = min(level_cnt[L], nxt_cap);
+    const int lane = threadIdx.x & 63;
+    const uint32_t stride = gridDim.x * 256;
+    const uint32_t lim = (n + 63u) & ~63u;
+    uint32_t nb = 0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < lim; i += stride) {
+        const bool valid = i < n;
+        RangeRec rec;
+        int flags = 0;
+        if (valid) {
+            rec = cur[i];
+            flags = bound_rec(objs, views, prm, rec);
+        }
+        const uint64_t sb = __ballot(flags & 2);
+        const int ns = __popcll(sb);
+        if (ns) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&level_cnt[L + 1], 2u * (uint32_t)ns);
+            base = __shfl(base, 0);
+            if (flags & 2) {
+                const uint32_t k = base + 2u * lanes_below(sb);
+                if (k + 1 < nxt_cap) {
+                    RangeRec c0, c1;
+                    split_children(rec, flags, c0, c1);
+                    nxt[k] = c0;
+                    nxt[k + 1] = c1;
+                }
+            }
+            if (base + 2u * (uint32_t)ns > nxt_cap && lane == 0) atomicOr(&fc->overflow, 8u);
+        }
+        const uint64_t db = __ballot(flags & 1);
+        const int nd = __popcll(db);
+        if (nd) {
+            uint32_t dbase = 0;
+            if (lane == 0) dbase = atomicAdd(&fc->draw_count, (uint32_t)nd);
+            dbase = __shfl(dbase, 0);
+            const uint32_t idx = dbase + lanes_below(db);
+            const bool put = (flags & 1) && idx < cb.cap;
+            if (put) cb.draw[idx] = rec;
+            wave_count_add(cb.counts, put ? gpid_of(objs, rec) : 0u, put);
+            if (dbase + (uint32_t)nd > cb.cap && lane == 0) atomicOr(&fc->overflow, 1u);
+        }
+        nb += (uint32_t)__popcll(__ballot(valid));
+    }
+    if (lane == 0 && nb) atomicAdd(&fc->bounds, (unsigned long long)nb);
+}
+
+// Leftover ranges after the last launched level go to the persistent
+// engine's private queues (round-robin).
+__global__ void k_level_to_queues(const RangeRec* __restrict__ list, const uint32_t* level_cnt, int L,
+                                  SplitQueues q, uint32_t list_cap)
+{
+    const uint32_t n = min(level_cnt[L], list_cap);
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t G = (uint32_t)q.n_groups;
+    if (g < G) q.queue_cnt[g] = (int32_t)((n + G - 1u - g) / G);
+    for (uint32_t i = g; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = i / G;
+        if ((int64_t)slot < q.stride) q.queue[(int64_t)(i % G) * q.stride + slot] = list[i];
+    }
+}
+
+int level_grid(int num_cus)
+{
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_level, 256, 0) != hipSuccess || blocks < 1)
+        blocks = 1;
+    return blocks * num_cus;
+}
+
+hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView* views, const DevParams& prm,
+                               uint32_t total_roots, RangeRec* list_a, RangeRec* list_b, uint32_t list_cap,
+                               uint32_t* level_cnt, int n_levels, const SplitQueues& q,
+                               const CanonBuffers& cb, FrameCounters* fc, int grid, hipStream_t s)
+{
+    const uint32_t n0 = total_roots > (uint32_t)(n_levels + 1) ? total_roots : (uint32_t)(n_levels + 1);
+    hipLaunchKernelGGL(k_level_roots, dim3((n0 + 255) / 256), dim3(256), 0, s, objs, n_objs, total_roots,
+                       list_a, level_cnt, n_levels);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (int L = 0; L < n_levels; ++L) {
+        const RangeRec* cur = (L & 1) ? list_b : list_a;
+        RangeRec* nxt = (L & 1) ? list_a : list_b;
+        hipLaunchKernelGGL(k_level, dim3(grid), dim3(256), 0, s, objs, views, prm, cur, nxt, level_cnt, L,
+                           list_cap, cb, fc);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    // anything deeper finishes in the persistent engine
+    const RangeRec* left = (n_levels & 1) ? list_b : list_a;
+    const uint32_t G = (uint32_t)q.n_groups;
+    hipLaunchKernelGGL(k_level_to_queues, dim3((G + 255) / 256 > 64 ? (G + 255) / 256 : 64), dim3(256), 0, s,
+                       left, level_cnt, n_levels, q, list_cap);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_split_persistent(objs, views, prm, q, cb, fc, 0, s);
 }
 
 }  // namespace mp

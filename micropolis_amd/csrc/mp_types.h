@@ -43,6 +43,8 @@ struct DevView {
     uint32_t range_base;         // canonical index of its first range (device-written)
     uint32_t range_count;        // device-written
     uint32_t gpid_end;           // one past the global patch index of its last object
+    uint32_t pixel_offset;       // first pixel of this view in multi-view buffers
+    uint32_t pad_;
 };
 
 // One draw_patches call: patch buffer + modelview + colour.
@@ -56,6 +58,8 @@ struct DevObject {
     int32_t n_patches;
     int32_t view;
     uint32_t gpid_base;          // global patch index of patch 0
+    uint32_t grad_offset;        // float offset of its control-point gradients
+    int32_t draw_index;          // position in draw-call order
 };
 
 struct DevParams {

@@ -174,6 +174,16 @@ class Renderer:
         self._check(self._lib.mp_get_stats(self._ctx, C.byref(st)))
         return {name: getattr(st, name) for name, _ in mp_stats._fields_}
 
+    # -- differentiable extension (device pointers; see include/micropolis.h) --
+    def backward(self, dl_dcolor_ptr, grad_cps_ptr, grad_color_ptr, grad_amp_ptr):
+        self._check(self._lib.mp_backward(self._ctx, dl_dcolor_ptr, grad_cps_ptr, grad_color_ptr, grad_amp_ptr))
+
+    def read_device(self, view, rgba32_ptr=None, ids_ptr=None):
+        self._check(self._lib.mp_batch_read_device(self._ctx, int(view), rgba32_ptr, ids_ptr))
+
+    def update_patches_device(self, handle, xyz_ptr, n_points):
+        self._check(self._lib.mp_update_patches_device(self._ctx, int(handle), xyz_ptr, int(n_points)))
+
     def replay_stages(self, n):
         ms = (C.c_double * 4)()
         self._check(self._lib.mp_replay_stages(self._ctx, int(n), ms))

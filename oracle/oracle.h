@@ -153,6 +153,16 @@ void or_resolve(const uint64_t* keybuf, const float* quad_store, const float* ra
 void or_make_view(float fovy_deg, float near_, int width, int height,
                   const float clear_color[4], or_view* out);
 
+/* FP64 loss and gradients with visibility fixed to ids (adjoint.c).
+ * cps_xyz: n_patches*16*3 floats (Bezier); ranges: the view's canonical
+ * DRAW ranges (ids index them); target_rgb: W*H*3 linear colour.  Gradients
+ * are accumulated into the (optional) outputs; returns the loss. */
+double or_adjoint_f64(const float* cps_xyz, int n_patches, const or_range* ranges, long n_ranges,
+                      const float mv[16], const or_params* prm, double amplitude,
+                      const double albedo[3], const uint32_t* ids, const float* target_rgb,
+                      int W, int H, double* grad_cps, double* grad_albedo, double* grad_amp,
+                      double* color_out);
+
 int  or_framebuffer_index(int x, int y, int width);
 long or_framebuffer_size(int width, int height);
 

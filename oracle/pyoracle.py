@@ -81,6 +81,8 @@ def load():
         "or_to_srgb8": (C.c_uint8, [C.c_float]),
         "or_prefix_sum": (C.c_int32, [_P, _P, C.c_long]),
         "or_glibc_rand": (None, [C.c_uint32, _P, C.c_long]),
+        "or_adjoint_f64": (C.c_double, [_FP, C.c_int, _P, C.c_long, _FP, _P, C.c_double, _P, _P, _P,
+                                        C.c_int, C.c_int, _P, _P, _P, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -224,3 +226,26 @@ def render(workload, method=None, want_keys=False):
             out["keys"] = fb
         results.append(out)
     return results
+
+
+def adjoint_f64(patches, ranges, mv, prm, amplitude, albedo, ids, target_rgb, grads=True):
+    """FP64 loss (and gradients) with visibility fixed to `ids`.
+    Returns (loss, grad_cps (P,16,3), grad_albedo (3,), grad_amp)."""
+    lib = load()
+    xyz = np.ascontiguousarray(np.asarray(patches, np.float32).reshape(-1, 16, 3))
+    r = np.ascontiguousarray(ranges)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    H, W = ids.shape
+    tgt = np.ascontiguousarray(np.asarray(target_rgb, np.float32).reshape(H, W, 3))
+    alb = np.ascontiguousarray(np.asarray(albedo, np.float64).reshape(3))
+    g_cps = np.zeros(xyz.shape, np.float64) if grads else None
+    g_alb = np.zeros(3, np.float64) if grads else None
+    g_amp = np.zeros(1, np.float64) if grads else None
+    mvc = colmajor(mv)
+    loss = lib.or_adjoint_f64(fptr(xyz), xyz.shape[0], r.ctypes.data, len(r), fptr(mvc), C.byref(prm),
+                              float(amplitude), alb.ctypes.data, ids.ctypes.data, tgt.ctypes.data, W, H,
+                              g_cps.ctypes.data if grads else None, g_alb.ctypes.data if grads else None,
+                              g_amp.ctypes.data if grads else None, None)
+    if not grads:
+        return loss
+    return loss, g_cps, g_alb, float(g_amp[0])

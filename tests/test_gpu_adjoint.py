@@ -1,0 +1,112 @@
+"""GPU adjoint: gradients of an image loss w.r.t. control points, the
+displacement amplitude and the albedo (visibility fixed to the forward ids)
+against the FP64 CPU reference, and against Richardson-extrapolated central
+differences (eps = 1e-3) of the fixed-visibility loss.  Tolerance: 1e-3
+relative error."""
+import numpy as np
+import pytest
+
+from micropolis_amd import Renderer, scenes
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def setup_problem(n_views=2, width=480, height=272):
+    from micropolis_amd.autograd import render_views
+    w = scenes.c5_views(n_views=n_views, width=width, height=height)
+    views = [(v.projection, v.objects[0].matrix) for v in w.views]
+    base = w.views[0].objects[0]
+    opts = dict(w.options, write_rgba32=1)
+    rng = np.random.default_rng(0)
+    cps0 = torch.tensor(base.patches, device="cuda")
+    albedo0 = torch.tensor(base.color[:3], device="cuda")
+    tgt_r = Renderer(**opts)
+    tcps = cps0 + torch.tensor(rng.normal(0, 0.01, base.patches.shape).astype(np.float32), device="cuda")
+    target, _ = render_views(tgt_r, tcps, 0.025, albedo0 + 0.1, views)
+    r = Renderer(**opts)
+    cps = cps0.clone().requires_grad_(True)
+    amp = torch.tensor(0.02, device="cuda", requires_grad=True)
+    albedo = albedo0.clone().requires_grad_(True)
+    rgb, ids = render_views(r, cps, amp, albedo, views)
+    mask = (ids != -1).unsqueeze(-1).float()
+    loss = (((rgb - target) ** 2) * mask).sum()
+    loss.backward()
+    return dict(w=w, r=r, tgt_r=tgt_r, cps=cps, amp=amp, albedo=albedo, ids=ids, target=target, loss=loss,
+                opts=opts, views=views)
+
+
+def fp64_reference(oracle, pb):
+    w, r = pb["w"], pb["r"]
+    ranges = r.ranges()
+    prm = oracle.make_params(pb["opts"])
+    P = pb["cps"].detach().cpu().numpy()
+    g_cps = np.zeros(P.shape, np.float64)
+    g_alb = np.zeros(3)
+    g_amp = 0.0
+    loss = 0.0
+    ids = pb["ids"].cpu().numpy().view(np.uint32)
+    tgt = pb["target"].cpu().numpy()
+    for vi, v in enumerate(w.views):
+        rv = ranges[ranges["object"] == vi]
+        rr = np.zeros(len(rv), oracle.RANGE_DTYPE)
+        for f in ("pid", "depth", "umin", "vmin", "umax", "vmax"):
+            rr[f] = rv[f]
+        l, gc, ga, gm = oracle.adjoint_f64(P, rr, v.objects[0].matrix, prm, float(pb["amp"]),
+                                          pb["albedo"].detach().cpu().numpy().astype(np.float64),
+                                          ids[vi], tgt[vi])
+        loss += l
+        g_cps += gc
+        g_alb += ga
+        g_amp += gm
+    return loss, g_cps, g_alb, g_amp
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_adjoint_matches_fp64_reference(oracle):
+    pb = setup_problem()
+    loss64, g_cps, g_alb, g_amp = fp64_reference(oracle, pb)
+    print("loss gpu %.6f fp64 %.6f" % (float(pb["loss"]), loss64), flush=True)
+    assert abs(float(pb["loss"]) - loss64) <= 1e-4 * loss64
+    assert np.abs(g_cps).max() > 0
+    assert rel(pb["cps"].grad.cpu().numpy(), g_cps) <= 1e-3
+    assert rel(pb["albedo"].grad.cpu().numpy(), g_alb) <= 1e-3
+    assert abs(float(pb["amp"].grad) - g_amp) <= 1e-3 * abs(g_amp)
+
+
+def test_adjoint_matches_central_differences(oracle):
+    """Richardson-extrapolated central differences (eps = 1e-3) of the FP64
+    fixed-visibility loss, on the coordinates with the largest gradients."""
+    pb = setup_problem(n_views=1)
+    w, r = pb["w"], pb["r"]
+    ranges = r.ranges()
+    rr = np.zeros(len(ranges), oracle.RANGE_DTYPE)
+    for f in ("pid", "depth", "umin", "vmin", "umax", "vmax"):
+        rr[f] = ranges[f]
+    prm = oracle.make_params(pb["opts"])
+    P = pb["cps"].detach().cpu().numpy().astype(np.float64)
+    ids = pb["ids"].cpu().numpy().view(np.uint32)[0]
+    tgt = pb["target"].cpu().numpy()[0]
+    alb = pb["albedo"].detach().cpu().numpy().astype(np.float64)
+    mv = w.views[0].objects[0].matrix
+
+    def L(x):
+        return oracle.adjoint_f64(x.astype(np.float32), rr, mv, prm, float(pb["amp"]), alb, ids, tgt, grads=False)
+
+    g = pb["cps"].grad.cpu().numpy().ravel()
+    for i in np.argsort(np.abs(g))[-6:]:
+        x0 = np.float32(P.ravel()[i])
+        e = float(np.float32(x0 + np.float32(1e-3)) - x0)
+
+        def f(h, i=i):
+            x = P.copy().ravel()
+            x[i] += h
+            return L(x.reshape(P.shape))
+        c1 = (f(e) - f(-e)) / (2 * e)
+        c2 = (f(e / 2) - f(-e / 2)) / e
+        fd = (4 * c2 - c1) / 3
+        assert abs(fd - g[i]) <= 1e-3 * abs(fd), (i, fd, g[i])
