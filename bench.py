@@ -180,6 +180,12 @@ def main():
     alg_bytes = st["diced_ranges"] * (24 + 4 * N * N) + P * 16 * 16 + st["coverage_samples"] * 8
     raster_ms = stages["raster"]
     achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "k_raster16_c2_torus_1080p.md")
+    if os.path.exists(prof):
+        for ln in open(prof):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                traffic = int(ln.split(":")[1])
 
     batch = None if args.no_batch else measure_batch(max(3, min(args.steps, 10)), use_graph)
     adjoint = None if args.no_adjoint else run_adjoint_child(max(3, min(args.steps, 10)))
@@ -220,7 +226,7 @@ def main():
                    "coverage_samples": st["coverage_samples"]},
         "stages_ms": {k: round(x, 4) for k, x in stages.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_raster<16>", "alg_bytes": int(alg_bytes)},
         "cpu_baseline": cpu,
         "batch": batch,
