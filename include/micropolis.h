@@ -161,9 +161,24 @@ int mp_read_ranges(mp_context* ctx, mp_range_record* records, int64_t capacity,
                    int64_t* count);
 
 /* Debug tap: per-pass DRAW counts of the last frame (BREADTH: per split
- * level; BOUNDED: per batch; LOCAL/BALANCED: per persistent launch). */
+ * level; BOUNDED and CPU: per batch of reyes_patches_per_pass; LOCAL/BALANCED:
+ * the DRAW total in slices of reyes_patches_per_pass). */
 int mp_read_pass_counts(mp_context* ctx, int32_t* counts, int32_t capacity,
                         int32_t* n_passes);
+
+/* Reyes::BoundNSplitCLCPU (src/Reyes/BoundNSplitCLCPU.cpp:31-159): the CPU
+ * method's host split of one patch set run to completion, with no context
+ * and no device (the split a CPU-method frame performs per object).
+ * records receive every DRAW range in canonical (pid, umin, vmin) order
+ * (object 0); pass NULL to query *count.  pass_counts receive the DRAW
+ * count of each batch of cfg->reyes_patches_per_pass that the reference's
+ * single stack would emit; *n_bounds the ranges bounded.  Any output
+ * pointer may be NULL.  On failure mp_create_error() holds the reason. */
+int mp_cpu_bound_n_split(const mp_config* cfg, const float* xyz, size_t n_points,
+                         int32_t patch_type, const float mv[16],
+                         const mp_projection* projection, mp_range_record* records,
+                         int64_t capacity, int64_t* count, int32_t* pass_counts,
+                         int32_t max_passes, int32_t* n_passes, int64_t* n_bounds);
 
 /* Statistics (src/base/Statistics.cpp:160-218) of the last frame. */
 int mp_get_stats(mp_context* ctx, mp_stats* stats);

@@ -95,6 +95,10 @@ EXPORTS = [
     ("mp_backward", C.c_int, [_P, _P, _P, _P, _P]),
     ("mp_batch_read_device", C.c_int, [_P, C.c_int32, _P, _P]),
     ("mp_update_patches_device", C.c_int, [_P, C.c_uint64, _P, C.c_size_t]),
+    ("mp_cpu_bound_n_split", C.c_int, [C.POINTER(mp_config), _FP, C.c_size_t, C.c_int32, _FP,
+                                       C.POINTER(mp_projection), C.POINTER(mp_range_record), C.c_int64,
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int32,
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
 ]
 
 _lib = None

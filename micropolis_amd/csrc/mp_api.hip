@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/micropolis.h"
+#include "mp_cpu_split.h"
 #include "mp_kernels.h"
 #include "mp_types.h"
 
@@ -36,6 +37,7 @@ struct PatchSet {
     float4* d_cps = nullptr;
     int type = 0;
     int n_patches = 0;
+    std::vector<float> host;   // x,y,z,1 per point; kept for the CPU split method
 };
 
 struct ObjectReq {
@@ -133,6 +135,9 @@ struct mp_context {
     int level_grid_size = 0;
     int spill_at = 192;
     int linger = -1;             // groups that wait for spilled work (-1: one per CU)
+    // CPU split method
+    CpuSplitFrame cpu_frame;
+    int cpu_threads = 0;         // 0: the host's cores (at most 16)
 };
 
 #define HIPCHK(ctx, expr)                                                                  \
@@ -246,10 +251,6 @@ static int validate_config(const mp_config* c, std::string& why)
         why = "unknown bound_n_split_method";
         return 0;
     }
-    if (c->bound_n_split_method == MP_BNS_CPU) {
-        why = "bound_n_split_method CPU is not available in this library";
-        return 0;
-    }
     if (c->window_width < 1 || c->window_height < 1) { why = "window_size must be positive"; return 0; }
     if (c->displacement && c->displacement_octaves < 1) { why = "displacement_octaves must be >= 1"; return 0; }
     return 1;
@@ -341,6 +342,7 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     if (const char* e = getenv("MP_LINGER")) ctx->linger = atoi(e);
     if (const char* e = getenv("MP_STACK_CAP")) ctx->stack_cap = std::max(256, atoi(e));
     if (const char* e = getenv("MP_SPLIT_ENGINE")) ctx->split_engine = atoi(e);
+    if (const char* e = getenv("MP_CPU_THREADS")) ctx->cpu_threads = atoi(e);
     ctx->level_launches = cfg->max_split_depth;
     if (const char* e = getenv("MP_RASTER_PER_CU")) ctx->raster_grid_size = ctx->num_cus * std::max(1, atoi(e));
     memset(&ctx->stats, 0, sizeof(ctx->stats));
@@ -410,6 +412,11 @@ extern "C" int mp_load_patches(mp_context* ctx, uint64_t handle, const float* xy
     PatchSet ps;
     ps.type = patch_type;
     ps.n_patches = (int)(n_points / per);
+    if (ctx->cfg.bound_n_split_method == MP_BNS_CPU) {
+        // BoundNSplitCLCPU keeps the host vector (PatchIndex::enable_retain_vector)
+        ps.host.resize(n_points * 4);
+        memcpy(ps.host.data(), host.data(), n_points * sizeof(float4));
+    }
     if (n_points) {
         HIPCHK(ctx, hipMalloc((void**)&ps.d_cps, n_points * sizeof(float4)));
         HIPCHK(ctx, hipMemcpy(ps.d_cps, host.data(), n_points * sizeof(float4), hipMemcpyHostToDevice));
@@ -876,6 +883,47 @@ static int run_segment_split(mp_context* ctx, uint32_t& draw_total, int64_t& dro
     return MP_OK;
 }
 
+// CPU method: host depth-first split on the host cores, DRAW list and
+// per-patch counts uploaded for the device back end
+static int run_cpu_split(mp_context* ctx, uint32_t& draw_total, int64_t& max_patches)
+{
+    const int no = (int)ctx->objects.size();
+    std::vector<CpuSplitObject> objs(no);
+    for (int oi = 0; oi < no; ++oi) {
+        const DevObject& d = ctx->h_objs[oi];
+        const PatchSet& ps = ctx->patches[ctx->objects[d.draw_index].handle];
+        const DevView& v = ctx->h_views[d.view];
+        CpuSplitObject& o = objs[oi];
+        o.cps = ps.host.data();
+        o.type = d.type;
+        o.n_patches = d.n_patches;
+        memcpy(o.mv, d.mv, sizeof(o.mv));
+        o.width = v.width;
+        o.height = v.height;
+        o.fx = v.f[0];
+        o.fy = v.f[1];
+        o.near_ = v.near_;
+        o.split_limit = d.split_limit;
+        o.gpid_base = d.gpid_base;
+        o.slot = (uint32_t)oi;
+    }
+    CpuSplitFrame& fr = ctx->cpu_frame;
+    cpu_split_frame(objs, ctx->n_gpids, ctx->cfg.cull_ribbon, ctx->cfg.max_split_depth,
+                    ctx->cfg.reyes_patches_per_pass, ctx->cpu_threads, fr);
+    draw_total = (uint32_t)fr.draws.size();
+    int rc;
+    if ((rc = grow_lists(ctx, std::max<uint32_t>(draw_total, 1), 0))) return rc;
+    if (draw_total)
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_draw, fr.draws.data(), (size_t)draw_total * sizeof(RangeRec),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->n_gpids)
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_counts, fr.counts.data(), (size_t)ctx->n_gpids * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    ctx->pass_counts = fr.pass_counts;
+    max_patches = fr.max_stack;
+    return MP_OK;
+}
+
 static int execute_frame(mp_context* ctx)
 {
     int rc;
@@ -939,6 +987,12 @@ static int execute_frame(mp_context* ctx)
             }
             ctx->level_launches = std::min(ctx->cfg.max_split_depth, deepest + 2);
         }
+    } else if (method == MP_BNS_CPU) {
+        HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
+        if ((rc = run_cpu_split(ctx, draw_total, max_patches))) return rc;
+        if ((rc = launch_back_end(ctx))) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     } else {
         HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
         if ((rc = run_segment_split(ctx, draw_total, dropped, max_patches))) return rc;
@@ -957,7 +1011,7 @@ static int execute_frame(mp_context* ctx)
     st.diced_ranges = ctx->h_fc->total_ranges;
     st.dropped_ranges = dropped;
     st.patches_per_frame = st.diced_ranges + dropped;
-    st.bounded_patches = (int64_t)ctx->h_fc->bounds;
+    st.bounded_patches = method == MP_BNS_CPU ? ctx->cpu_frame.bounds : (int64_t)ctx->h_fc->bounds;
     st.micropolygons = ctx->cfg.dummy_render ? 0 : N2 * st.diced_ranges;
     st.pass_count = (int64_t)ctx->pass_counts.size();
     st.max_patches = max_patches;
@@ -967,6 +1021,66 @@ static int execute_frame(mp_context* ctx)
     st.frame_ms = ms;
     ctx->device_driven = device_driven;
     ctx->frame_valid = true;
+    return MP_OK;
+}
+
+extern "C" int mp_cpu_bound_n_split(const mp_config* cfg, const float* xyz, size_t n_points,
+                                    int32_t patch_type, const float mv[16],
+                                    const mp_projection* projection, mp_range_record* records,
+                                    int64_t capacity, int64_t* count, int32_t* pass_counts,
+                                    int32_t max_passes, int32_t* n_passes, int64_t* n_bounds)
+{
+    if (!cfg || !mv || !projection) { g_create_error = "null argument"; return MP_E_INVALID; }
+    std::string why;
+    if (!validate_config(cfg, why)) { g_create_error = why; return MP_E_INVALID; }
+    if (patch_type != MP_BEZIER && patch_type != MP_GREGORY) { g_create_error = "unknown patch type"; return MP_E_INVALID; }
+    const size_t per = patch_type == MP_GREGORY ? 20 : 16;
+    if (n_points % per != 0 || (n_points && !xyz) || n_points / per >= (1u << 20)) {
+        g_create_error = "bad patch data";
+        return MP_E_INVALID;
+    }
+    if (projection->width < 1 || projection->height < 1) { g_create_error = "view size must be positive"; return MP_E_INVALID; }
+    std::vector<float> host(n_points * 4);
+    for (size_t i = 0; i < n_points; ++i) {
+        host[4 * i] = xyz[3 * i]; host[4 * i + 1] = xyz[3 * i + 1]; host[4 * i + 2] = xyz[3 * i + 2];
+        host[4 * i + 3] = 1.0f;
+    }
+    DevView v;
+    make_view(*projection, cfg->clear_color, cfg->subpixel_bits, v);
+    CpuSplitObject o;
+    o.cps = host.data();
+    o.type = patch_type;
+    o.n_patches = (int)(n_points / per);
+    memcpy(o.mv, mv, sizeof(o.mv));
+    o.width = v.width; o.height = v.height;
+    o.fx = v.f[0]; o.fy = v.f[1]; o.near_ = v.near_;
+    o.split_limit = cfg->bound_n_split_limit;
+    o.gpid_base = 0;
+    o.slot = 0;
+    CpuSplitFrame fr;
+    const char* e = getenv("MP_CPU_THREADS");
+    cpu_split_frame(std::vector<CpuSplitObject>(1, o), (uint32_t)o.n_patches, cfg->cull_ribbon,
+                    cfg->max_split_depth, cfg->reyes_patches_per_pass, e ? atoi(e) : 0, fr);
+    // canonical order: (pid, umin, vmin); umin, vmin are dyadic, so compare as floats
+    std::sort(fr.draws.begin(), fr.draws.end(), [](const RangeRec& a, const RangeRec& b) {
+        if (a.pid != b.pid) return a.pid < b.pid;
+        if (a.umin != b.umin) return a.umin < b.umin;
+        return a.vmin < b.vmin;
+    });
+    const int64_t n = (int64_t)fr.draws.size();
+    if (count) *count = n;
+    if (records) {
+        if (capacity < n) { g_create_error = "record capacity too small"; return MP_E_INVALID; }
+        for (int64_t i = 0; i < n; ++i) {
+            const RangeRec& r = fr.draws[(size_t)i];
+            records[i] = mp_range_record{ (int32_t)r.pid, 0, (int32_t)(r.obj_depth >> 24),
+                                          r.umin, r.vmin, r.umax, r.vmax };
+        }
+    }
+    if (n_passes) *n_passes = (int32_t)fr.pass_counts.size();
+    if (pass_counts)
+        for (int32_t k = 0; k < max_passes && k < (int32_t)fr.pass_counts.size(); ++k) pass_counts[k] = fr.pass_counts[k];
+    if (n_bounds) *n_bounds = fr.bounds;
     return MP_OK;
 }
 
@@ -1101,7 +1215,13 @@ extern "C" int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* 
             int64_t dropped = 0, maxp = 0;
             uint32_t dt = 0;
             HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
-            if ((rc = run_segment_split(ctx, dt, dropped, maxp))) return rc;
+            if (ctx->cfg.bound_n_split_method == MP_BNS_CPU) {
+                HIPCHK(ctx, hipStreamSynchronize(ctx->stream));   // the host lists are reused
+                rc = run_cpu_split(ctx, dt, maxp);
+            } else {
+                rc = run_segment_split(ctx, dt, dropped, maxp);
+            }
+            if (rc) return rc;
             if ((rc = launch_back_end(ctx))) return rc;
         }
     } else if (use_graph) {

@@ -364,9 +364,10 @@ MP_DEV void split_children(const RangeRec& r, int flags, RangeRec& c0, RangeRec&
 
 MP_DEV uint64_t canonical_key(const RangeRec& r)
 {
-    uint64_t u = (uint64_t)(uint32_t)(r.umin * 4194304.0f);
-    uint64_t v = (uint64_t)(uint32_t)(r.vmin * 4194304.0f);
-    return (u << 22) | v;
+    // umin, vmin are dyadic with at most 24 halvings per axis: exact below 2^24
+    uint64_t u = (uint64_t)(uint32_t)(r.umin * 16777216.0f);
+    uint64_t v = (uint64_t)(uint32_t)(r.vmin * 16777216.0f);
+    return (u << 24) | v;
 }
 
 // monotone uint32 image of a float (total order consistent with <)

@@ -45,24 +45,61 @@ def _colmajor(m):
     return np.ascontiguousarray(m.reshape(16))
 
 
+def make_config(**options):
+    """mp_config from reyes.options names (unset options keep their defaults)."""
+    cfg = _lib.default_config()
+    for key, val in options.items():
+        if key == "window_size":
+            cfg.window_width, cfg.window_height = int(val[0]), int(val[1])
+        elif key == "clear_color":
+            for i in range(4):
+                cfg.clear_color[i] = float(val[i])
+        elif key == "bound_n_split_method":
+            cfg.bound_n_split_method = METHODS[val] if isinstance(val, str) else int(val)
+        elif hasattr(cfg, key):
+            setattr(cfg, key, type(getattr(cfg, key))(val))
+        else:
+            raise MicropolisError("unknown option %r" % key)
+    return cfg
+
+
+RANGE_DTYPE = np.dtype([("pid", "<i4"), ("object", "<i4"), ("depth", "<i4"), ("umin", "<f4"),
+                        ("vmin", "<f4"), ("umax", "<f4"), ("vmax", "<f4")])
+
+
+def cpu_bound_n_split(patches, patch_type, matrix, projection, **options):
+    """The CPU method's host split of one patch set (Reyes::BoundNSplitCLCPU),
+    run to completion without a device.  Returns (DRAW ranges in canonical
+    order as a RANGE_DTYPE array, per-batch DRAW counts, ranges bounded)."""
+    lib = _lib.load()
+    cfg = make_config(**options)
+    xyz = np.ascontiguousarray(np.asarray(patches, np.float32).reshape(-1, 3))
+    mv = _colmajor(matrix)
+    pr = projection.as_c()
+    n = C.c_int64()
+    npass = C.c_int32()
+    nb = C.c_int64()
+    rc = lib.mp_cpu_bound_n_split(C.byref(cfg), _fptr(xyz), xyz.shape[0], int(patch_type), _fptr(mv),
+                                  C.byref(pr), None, 0, C.byref(n), None, 0, C.byref(npass), None)
+    if rc != 0:
+        raise MicropolisError(lib.mp_create_error().decode())
+    out = np.zeros(n.value, RANGE_DTYPE)
+    passes = np.zeros(max(1, npass.value), np.int32)
+    rc = lib.mp_cpu_bound_n_split(C.byref(cfg), _fptr(xyz), xyz.shape[0], int(patch_type), _fptr(mv),
+                                  C.byref(pr), out.ctypes.data_as(C.POINTER(mp_range_record)), n.value,
+                                  C.byref(n), passes.ctypes.data_as(C.POINTER(C.c_int32)), passes.size,
+                                  C.byref(npass), C.byref(nb))
+    if rc != 0:
+        raise MicropolisError(lib.mp_create_error().decode())
+    return out, passes[:npass.value].copy(), nb.value
+
+
 class Renderer:
     """Reyes::RendererCL over libmicropolis_hip.so."""
 
     def __init__(self, **options):
         lib = _lib.load()
-        cfg = _lib.default_config()
-        for key, val in options.items():
-            if key == "window_size":
-                cfg.window_width, cfg.window_height = int(val[0]), int(val[1])
-            elif key == "clear_color":
-                for i in range(4):
-                    cfg.clear_color[i] = float(val[i])
-            elif key == "bound_n_split_method":
-                cfg.bound_n_split_method = METHODS[val] if isinstance(val, str) else int(val)
-            elif hasattr(cfg, key):
-                setattr(cfg, key, type(getattr(cfg, key))(val))
-            else:
-                raise MicropolisError("unknown option %r" % key)
+        cfg = make_config(**options)
         self.cfg = cfg
         self._lib = lib
         ctx = lib.mp_create(C.byref(cfg))

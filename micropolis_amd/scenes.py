@@ -245,15 +245,19 @@ def gregory_planes(rng, n_planes=32, nu=32, nv=16, width=4.0, height=2.0, spacin
     return np.array(pats, np.float32)
 
 
-def c4_gregory(seed=4, width=1920, height=1080):
+def c4_gregory(seed=4, width=1920, height=1080, planes=32, quads=512):
+    """32 stacked planes x 512 perturbed quads as Gregory patches (16384);
+    `planes` and `quads` (32 x 16 grid scaled down) shrink it for tests."""
     rng = np.random.default_rng(seed)
-    pats = gregory_planes(rng)
+    nu = 32 if quads >= 512 else max(2, int(round(np.sqrt(quads * 2))))
+    nv = max(1, quads // nu)
+    pats = gregory_planes(rng, n_planes=planes, nu=nu, nv=nv)
     mv = look_at((0.0, 0.0, 3.0), (0.0, 0.0, -0.6))
     opts = dict(window_size=(width, height), bound_n_split_method="BALANCED", bound_n_split_limit=8.0,
                 reyes_patch_size=8, local_bns_work_groups=0)
     proj = Projection(60.0, 0.01, (width, height))
     o = _obj(1, pats, GREGORY, mv, (0.6, 0.8, 0.9, 1.0))
-    return Workload("c4_gregory_16384p_%dx%d_balanced" % (width, height), opts, [View(proj, [o])])
+    return Workload("c4_gregory_%dp_%dx%d_balanced" % (len(pats), width, height), opts, [View(proj, [o])])
 
 
 WORKLOADS = {"c1": c1_sphere, "c2": c2_torus, "c3": c3_batch, "c4": c4_gregory, "c5": c5_views}

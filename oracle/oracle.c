@@ -22,6 +22,11 @@
  *                      srand(43), rand()%8+1) with glibc's rand sequence
  *   or_noise           lattice/interpolation invariants (values at lattice
  *                      points, continuity, octave sum bound)
+ *   or_bound_cpu       FP64 de Casteljau rows vs the glm-order evaluation,
+ *   or_eval_patch_cpu  analytic sizes of planar patches through
+ *   or_projection_bound Projection::bound, hand-built cull/drop cases, the
+ *   or_split (CPU)     split direction of elongated patches, brute-force
+ *                      recursion of the predicate, batch recurrence
  */
 #include "oracle.h"
 
@@ -378,6 +383,137 @@ int or_bound(const float* cps, int type, int pid, int depth,
     return (hlen > vlen) ? OR_VSPLIT : OR_HSPLIT;
 }
 
+/* ------------------------------------------------------------------------ */
+/* CPU bound&split method (src/Reyes/BoundNSplitCLCPU.cpp, src/base/Patch.cpp,
+ * src/base/Projection.cpp).  Host code built on glm; glm is not vendored in
+ * the reference, so the vector operations follow glm 0.9.9's published
+ * definitions: mat4 * vec4 = (c0*x + c1*y) + (c2*z + c3*w) per row,
+ * dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z, length = sqrt(dot(v,v)),
+ * distance(a,b) = length(b - a), scalar * vec and vec / scalar per component. */
+
+/* eval_spline (Patch.cpp:107-113): s*s*s*p0 + 3*s*s*t*p1 + 3*s*t*t*p2 + t*t*t*p3 */
+static void cpu_spline(const float* p0, const float* p1, const float* p2, const float* p3,
+                       float t, float* dst)
+{
+    float s = 1.0f - t;
+    float a = (s * s) * s, b = ((3.0f * s) * s) * t, c = ((3.0f * s) * t) * t, d = (t * t) * t;
+    for (int k = 0; k < 3; ++k)
+        dst[k] = ((a * p0[k] + b * p1[k]) + c * p2[k]) + d * p3[k];
+}
+
+/* eval_patch (Patch.cpp:115-129): rows P[i][0..3] along t, then along s.
+ * cp holds 16 points of 4 floats (P[i][j] at 4i+j). */
+static void cpu_eval_bezier(const float* cp, float t, float s, float* dst)
+{
+    float p[4][3];
+    for (int i = 0; i < 4; ++i)
+        cpu_spline(cp + 4 * (4 * i + 0), cp + 4 * (4 * i + 1), cp + 4 * (4 * i + 2),
+                   cp + 4 * (4 * i + 3), t, p[i]);
+    cpu_spline(p[0], p[1], p[2], p[3], s, dst);
+}
+
+/* eval_gregory_patch (Patch.cpp:186-211): u and v swapped on entry; the F
+ * terms have no NaN guard, so every patch corner evaluates to NaN. */
+static void cpu_eval_gregory(const float* P, float u, float v, float* dst)
+{
+    float tmp = u; u = v; v = tmp;
+    const float *p = P, *ep = P + 16, *em = P + 32, *fp = P + 48, *fm = P + 64;
+    float F0[3], F1[3], F2[3], F3[3];
+    for (int k = 0; k < 3; ++k) {
+        F0[k] = (u * fp[0 + k] + v * fm[0 + k]) / (u + v);
+        F1[k] = ((1.0f - u) * fm[4 + k] + v * fp[4 + k]) / ((1.0f - u) + v);
+        F2[k] = ((1.0f - u) * fm[8 + k] + (1.0f - v) * fp[8 + k]) / ((2.0f - u) - v);
+        F3[k] = (u * fm[12 + k] + (1.0f - v) * fp[12 + k]) / ((1.0f + u) - v);
+    }
+    float R[4][3];
+    cpu_spline(p + 0,  em + 0, ep + 12, p + 12, u, R[0]);
+    cpu_spline(ep + 0, F0,     F3,      em + 12, u, R[1]);
+    cpu_spline(em + 4, F1,     F2,      ep + 8, u, R[2]);
+    cpu_spline(p + 4,  ep + 4, em + 8,  p + 8,  u, R[3]);
+    cpu_spline(R[0], R[1], R[2], R[3], v, dst);
+}
+
+void or_eval_patch_cpu(const float* cps, int type, int pid, float u, float v, float out[3])
+{
+    if (type == OR_GREGORY) cpu_eval_gregory(cps + (long)pid * 20 * 4, u, v, out);
+    else                    cpu_eval_bezier(cps + (long)pid * 16 * 4, u, v, out);
+}
+
+static float glm_distance(const float* a, const float* b)
+{
+    float d[3] = { b[0] - a[0], b[1] - a[1], b[2] - a[2] };
+    return sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+}
+
+/* Projection::bound (Projection.cpp:57-106).  Returns 1 when culled. */
+int or_projection_bound(const float bmin[3], const float bmax[3], const or_view* P,
+                        float cull_ribbon, float size[2])
+{
+    float n = -bmax[2];
+    float f = -bmin[2];
+    if (f < P->near_) return 1;
+    n = (P->near_ > n) ? P->near_ : n;          /* maximum(_near, n) */
+    const float fx = P->f[0], fy = P->f[1];
+    const float vpx = (float)P->width / 2.0f, vpy = (float)P->height / 2.0f;
+    float mnx = (bmin[0] < 0) ? bmin[0] / n * fx + vpx : bmin[0] / f * fx + vpx;
+    float mny = (bmin[1] < 0) ? bmin[1] / n * fy + vpy : bmin[1] / f * fy + vpy;
+    float mxx = (bmax[0] > 0) ? bmax[0] / n * fx + vpx : bmax[0] / f * fx + vpx;
+    float mxy = (bmax[1] > 0) ? bmax[1] / n * fy + vpy : bmax[1] / f * fy + vpy;
+    if (mnx > (float)(P->width - 1) + cull_ribbon || mxx < -cull_ribbon ||
+        mny > (float)(P->height - 1) + cull_ribbon || mxy < -cull_ribbon)
+        return 1;
+    size[0] = mxx - mnx;
+    size[1] = mxy - mny;
+    return 0;
+}
+
+/* One step of BoundNSplitCLCPU::do_bound_n_split (:102-148) for one range:
+ * bound_patch_range / bound_gregory_patch_range (:249-336) with RES = 3,
+ * Projection::bound, then the DRAW / drop / split decision.  Returns OR_CULL,
+ * OR_DRAW, OR_CPU_DROP (depth > max_split_depth), OR_HSPLIT (halve the
+ * range's second coordinate: vsplit_range) or OR_VSPLIT (halve the first:
+ * hsplit_range). */
+int or_bound_cpu(const float* cps, int type, int pid, int depth,
+                 float umin, float vmin, float umax, float vmax,
+                 const float mv[16], const or_view* view, const or_params* prm, float split_limit)
+{
+    float ps[3][3][3];
+    float bmin[3] = { INFINITY, INFINITY, INFINITY };
+    float bmax[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (int iu = 0; iu < 3; ++iu) {
+        for (int iv = 0; iv < 3; ++iv) {
+            /* (max - min) * (size_t)i * (1.0f / (RES-1)) */
+            float v = umin + ((umax - umin) * (float)iu) * 0.5f;
+            float u = vmin + ((vmax - vmin) * (float)iv) * 0.5f;
+            float pos[3];
+            if (type == OR_GREGORY) cpu_eval_gregory(cps + (long)pid * 20 * 4, u, v, pos);
+            else                    cpu_eval_bezier(cps + (long)pid * 16 * 4, u, v, pos);
+            float* pt = ps[iu][iv];
+            for (int r = 0; r < 3; ++r)
+                pt[r] = (mv[r] * pos[0] + mv[4 + r] * pos[1]) + (mv[8 + r] * pos[2] + mv[12 + r] * 1.0f);
+            for (int k = 0; k < 3; ++k) {            /* BBox::add_point */
+                bmin[k] = (pt[k] < bmin[k]) ? pt[k] : bmin[k];
+                bmax[k] = (pt[k] > bmax[k]) ? pt[k] : bmax[k];
+            }
+        }
+    }
+    float vlen = 0.0f, hlen = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+        float h = 0.0f, v = 0.0f;
+        for (int j = 0; j < 2; ++j) {
+            v = v + glm_distance(ps[j][i], ps[j + 1][i]);
+            h = h + glm_distance(ps[i][j], ps[i][j + 1]);
+        }
+        vlen = (v > vlen) ? v : vlen;
+        hlen = (h > hlen) ? h : hlen;
+    }
+    float size[2];
+    if (or_projection_bound(bmin, bmax, view, prm->cull_ribbon, size)) return OR_CULL;
+    if (bmin[2] < 0 && size[0] < split_limit && size[1] < split_limit) return OR_DRAW;
+    if (depth > prm->max_split_depth) return OR_CPU_DROP;
+    return (vlen < hlen) ? OR_HSPLIT : OR_VSPLIT;
+}
+
 /* Children of a split (bound_n_split_breadthfirst.cl:99-113). */
 static void split_range(const or_range* r, int flags, or_range* c0, or_range* c1)
 {
@@ -498,10 +634,44 @@ long or_split(int method, const float* cps, int type, int n_patches,
             ++passes;
         }
         free(st.d); free(pad); free(flg);
+    } else if (method == OR_METHOD_CPU) {
+        /* Host DFS of BoundNSplitCLCPU::do_bound_n_split (:75-159): the stack
+         * holds the roots in patch order and pops from the back; a batch ends
+         * when it holds reyes_patches_per_pass DRAW ranges (:135-137). */
+        rvec st = { 0 };
+        for (int i = 0; i < n_patches; ++i) {
+            or_range r = { i, 0, 0.0f, 0.0f, 1.0f, 1.0f };
+            rpush(&st, r);
+        }
+        while (st.n > 0) {
+            long draws = 0;
+            while (st.n > 0) {
+                or_range r = st.d[--st.n];
+                int fl = or_bound_cpu(cps, type, r.pid, r.depth, r.umin, r.vmin, r.umax, r.vmax,
+                                      mv, view, prm, split_limit);
+                ++nb;
+                if (fl == OR_CULL) continue;
+                if (fl == OR_DRAW) {
+                    if (nout >= out_cap) { free(st.d); return -1; }
+                    out[nout++] = r;
+                    if (++draws >= ppp) break;
+                } else if (fl == OR_CPU_DROP) {
+                    /* depth > max_split_depth: dropped (:139-141) */
+                } else {
+                    or_range c0, c1;
+                    split_range(&r, fl, &c0, &c1);
+                    rpush(&st, c0);   /* lower half first: the upper half */
+                    rpush(&st, c1);   /* is popped next (:231-245)        */
+                }
+            }
+            if (passes < max_passes && pass_counts) pass_counts[passes] = (int32_t)draws;
+            ++passes;
+        }
+        free(st.d);
     } else {
-        /* LOCAL / BALANCED / CPU-less GPU methods: the DRAW multiset is the
-         * canonical set of the predicate (order is scheduling-dependent in the
-         * reference).  Depth-first traversal; one pass. */
+        /* LOCAL / BALANCED: the DRAW multiset is the canonical set of the
+         * predicate (order is scheduling-dependent in the reference).
+         * Depth-first traversal; one pass. */
         rvec st = { 0 };
         for (int i = n_patches - 1; i >= 0; --i) {
             or_range r = { i, 0, 0.0f, 0.0f, 1.0f, 1.0f };
@@ -641,17 +811,24 @@ done:
 
 /* Canonical order of DRAW ranges: (pid, umin, vmin).  The split tree's
  * leaves of one patch are disjoint dyadic rectangles, so (umin, vmin) is
- * unique per leaf; umin*2^22 is an exact integer for depth < 23. */
+ * unique per leaf.
+ * Key of a range inside its patch: umin and vmin are dyadic with at most 24
+ * halvings per axis (max_split_depth <= 23, the CPU method draws at depth
+ * max_split_depth + 1), so umin * 2^24 and vmin * 2^24 are exact integers
+ * below 2^24. */
 uint64_t or_canonical_key(const or_range* r)
 {
-    uint64_t u = (uint64_t)(uint32_t)(r->umin * 4194304.0f);
-    uint64_t v = (uint64_t)(uint32_t)(r->vmin * 4194304.0f);
-    return ((uint64_t)(uint32_t)r->pid << 44) | (u << 22) | v;
+    uint64_t u = (uint64_t)(uint32_t)(r->umin * 16777216.0f);
+    uint64_t v = (uint64_t)(uint32_t)(r->vmin * 16777216.0f);
+    return (u << 24) | v;
 }
 
 static int cmp_range(const void* a, const void* b)
 {
-    uint64_t ka = or_canonical_key((const or_range*)a), kb = or_canonical_key((const or_range*)b);
+    const or_range* ra = (const or_range*)a;
+    const or_range* rb = (const or_range*)b;
+    if (ra->pid != rb->pid) return ra->pid < rb->pid ? -1 : 1;
+    uint64_t ka = or_canonical_key(ra), kb = or_canonical_key(rb);
     return ka < kb ? -1 : (ka > kb ? 1 : 0);
 }
 

@@ -32,6 +32,7 @@ extern "C" {
 #define OR_DRAW   1
 #define OR_HSPLIT 2
 #define OR_VSPLIT 6
+#define OR_CPU_DROP 8   /* CPU method: depth > max_split_depth, range dropped */
 
 #define OR_BEZIER  0
 #define OR_GREGORY 1
@@ -92,11 +93,25 @@ int or_bound(const float* cps, int type, int pid, int depth,
              const float mv[16], const or_view* view, const or_params* prm,
              float split_limit);
 
+/* ---- CPU bound&split method (src/Reyes/BoundNSplitCLCPU.cpp:249-336,
+ * src/base/Patch.cpp:107-211, src/base/Projection.cpp:57-106) ----
+ * or_eval_patch_cpu: the host evaluation order (eval_spline rows, glm).
+ * or_projection_bound: screen size of an eye-space box; returns 1 if culled.
+ * or_bound_cpu: OR_CULL, OR_DRAW, OR_CPU_DROP, OR_HSPLIT (halve the second
+ * range coordinate) or OR_VSPLIT (halve the first). */
+void or_eval_patch_cpu(const float* cps, int type, int pid, float u, float v, float out[3]);
+int  or_projection_bound(const float bmin[3], const float bmax[3], const or_view* view,
+                         float cull_ribbon, float size[2]);
+int  or_bound_cpu(const float* cps, int type, int pid, int depth,
+                  float umin, float vmin, float umax, float vmax,
+                  const float mv[16], const or_view* view, const or_params* prm,
+                  float split_limit);
+
 /* ---- split schedules ----
  * Returns the number of DRAW ranges written to out (all passes concatenated
  * in emission order), or -1 if out_cap is too small.  pass_counts[k] is the
- * number of DRAW ranges emitted by pass k (BREADTH: level k; BOUNDED: batch k;
- * LOCAL/BALANCED: 1 pass with the full set), *n_passes the pass count,
+ * number of DRAW ranges emitted by pass k (BREADTH: level k; BOUNDED and CPU:
+ * batch k; LOCAL/BALANCED: 1 pass with the full set), *n_passes the pass count,
  * *n_bounds the number of bound() evaluations.  For BREADTH the ranges the
  * renderer drops (RendererCL.cpp:209) are NOT written; pass_counts holds the
  * untruncated counts and *n_dropped the dropped total. */
@@ -116,7 +131,8 @@ long or_split_local_emulated(const float* cps, int type, int n_patches,
                              or_range* out, long out_cap, int32_t* pass_counts, int max_passes,
                              int32_t* n_passes, long* n_bounds);
 
-/* Sort ranges into canonical order: (pid, umin, vmin). */
+/* Sort ranges into canonical order: (pid, umin, vmin).  or_canonical_key is
+ * the order inside one patch: (umin * 2^24) << 24 | vmin * 2^24. */
 void or_canonical_sort(or_range* r, long n);
 uint64_t or_canonical_key(const or_range* r);
 

@@ -17,7 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-CULL, DRAW, HSPLIT, VSPLIT = 0, 1, 2, 6
+CULL, DRAW, HSPLIT, VSPLIT, CPU_DROP = 0, 1, 2, 6, 8
 METHODS = {"CPU": 0, "BOUNDED": 1, "LOCAL": 2, "BREADTH": 3, "BALANCED": 4}
 EMPTY_KEY = 0xFFFFFFFFFFFFFFFF
 
@@ -63,6 +63,10 @@ def load():
         "or_displaced_point": (None, [_FP, C.c_int, C.c_int, C.c_float, C.c_float, _P, _FP]),
         "or_bound": (C.c_int, [_FP, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                C.c_float, _FP, _P, _P, C.c_float]),
+        "or_eval_patch_cpu": (None, [_FP, C.c_int, C.c_int, C.c_float, C.c_float, _FP]),
+        "or_projection_bound": (C.c_int, [_FP, _FP, _P, C.c_float, _FP]),
+        "or_bound_cpu": (C.c_int, [_FP, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                   C.c_float, _FP, _P, _P, C.c_float]),
         "or_split": (C.c_long, [C.c_int, _FP, C.c_int, C.c_int, _FP, _P, _P, C.c_float, _P,
                                 C.c_long, _P, C.c_int, _P, _P, _P]),
         "or_split_local_emulated": (C.c_long, [_FP, C.c_int, C.c_int, _FP, _P, _P, C.c_float,

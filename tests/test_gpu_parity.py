@@ -9,15 +9,17 @@ from gpu_util import assert_frame_equal, render_plain
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("method", ["BREADTH", "BOUNDED", "LOCAL", "BALANCED"])
+@pytest.mark.parametrize("method", ["BREADTH", "BOUNDED", "LOCAL", "BALANCED", "CPU"])
 def test_c1_sphere_all_methods(oracle, method):
     w = scenes.c1_sphere()
     print("case c1_sphere", method, flush=True)
     r, got = render_plain(w, bound_n_split_method=method, write_rgba32=1)
     ref = oracle.render(w, method=method)[0]
     assert_frame_equal(got, ref, "c1/" + method)
-    if method in ("BREADTH", "BOUNDED"):
+    if method in ("BREADTH", "BOUNDED", "CPU"):
         assert np.array_equal(got["pass_counts"], ref["pass_counts"])
+    if method == "CPU":
+        assert got["stats"]["bounded_patches"] == ref["n_bounds"]
     assert got["stats"]["dropped_ranges"] == ref["n_dropped"]
     r.close()
 
@@ -53,4 +55,20 @@ def test_replay_and_graph_identical(oracle):
         again = r.read_framebuffer()
         assert np.array_equal(again["ids"], got["ids"])
         assert np.array_equal(again["rgba8"], got["rgba8"])
+    r.close()
+
+
+def test_cpu_method_gregory_and_replay(oracle):
+    """CPU method on Gregory patches (NaN corner samples of the host
+    evaluation) through the device back end, and its host-driven replay."""
+    w = scenes.c4_gregory(planes=4, quads=128, width=640, height=360)
+    w.options["reyes_patches_per_pass"] = 2000
+    print("case c4_gregory small CPU", flush=True)
+    r, got = render_plain(w, bound_n_split_method="CPU")
+    ref = oracle.render(w, method="CPU")[0]
+    assert_frame_equal(got, ref, "c4-small/CPU")
+    assert np.array_equal(got["pass_counts"], ref["pass_counts"])
+    r.replay(2, use_graph=False)
+    again = r.read_framebuffer()
+    assert np.array_equal(again["ids"], got["ids"])
     r.close()
