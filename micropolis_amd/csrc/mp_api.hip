@@ -331,6 +331,8 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     // hashing the lattice per use measured faster than the per-range LDS cache
     ctx->prm.noise_cache = 0;
     if (const char* e = getenv("MP_NOISE_CACHE")) ctx->prm.noise_cache = atoi(e);
+    ctx->prm.noise_wave_cells = 1;
+    if (const char* e = getenv("MP_NOISE_WAVE_CELLS")) ctx->prm.noise_wave_cells = atoi(e);
     ctx->raster_grid_size = raster_grid(cfg->reyes_patch_size, ctx->num_cus, ctx->prm);
     // LDS ring stack of the persistent split, depth-bounded: 64 * (max_split_depth + 2)
     // entries.  Fewer resident groups that own their SIMD finish the deepest

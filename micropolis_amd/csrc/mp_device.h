@@ -151,18 +151,39 @@ MP_DEV float lattice(int32_t x, int32_t y, int32_t z, uint32_t seed)
 {
     return (float)(noise_hash(x, y, z, seed) >> 8) * (1.0f / 8388608.0f) - 1.0f;
 }
+// Same value for wave-uniform arguments; the readfirstlane keeps the hash on
+// the scalar unit (the compiler would otherwise merge it with the per-lane
+// path and run it on the vector unit).
+MP_DEV float lattice_wave(int32_t x, int32_t y, int32_t z, uint32_t seed)
+{
+    const uint32_t h = (uint32_t)__builtin_amdgcn_readfirstlane((int)noise_hash(x, y, z, seed));
+    return (float)(h >> 8) * (1.0f / 8388608.0f) - 1.0f;
+}
 MP_DEV float fade(float t) { return (t * t) * (3.0f - 2.0f * t); }
 MP_DEV float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
 
-MP_DEV float vnoise(float x, float y, float z, uint32_t seed)
+MP_DEV float vnoise(float x, float y, float z, uint32_t seed, bool wave_cells = false)
 {
     float fx = floorf(x), fy = floorf(y), fz = floorf(z);
     int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
     float wx = fade(x - fx), wy = fade(y - fy), wz = fade(z - fz);
-    float c000 = lattice(ix, iy, iz, seed),         c100 = lattice(ix + 1, iy, iz, seed);
-    float c010 = lattice(ix, iy + 1, iz, seed),     c110 = lattice(ix + 1, iy + 1, iz, seed);
-    float c001 = lattice(ix, iy, iz + 1, seed),     c101 = lattice(ix + 1, iy, iz + 1, seed);
-    float c011 = lattice(ix, iy + 1, iz + 1, seed), c111 = lattice(ix + 1, iy + 1, iz + 1, seed);
+    float c000, c100, c010, c110, c001, c101, c011, c111;
+    const int32_t jx = __builtin_amdgcn_readfirstlane(ix);
+    const int32_t jy = __builtin_amdgcn_readfirstlane(iy);
+    const int32_t jz = __builtin_amdgcn_readfirstlane(iz);
+    if (wave_cells && __all(ix == jx && iy == jy && iz == jz)) {
+        // every active lane is in the same lattice cell: the eight corner
+        // hashes are wave-uniform (scalar unit), the values are identical
+        c000 = lattice_wave(jx, jy, jz, seed);         c100 = lattice_wave(jx + 1, jy, jz, seed);
+        c010 = lattice_wave(jx, jy + 1, jz, seed);     c110 = lattice_wave(jx + 1, jy + 1, jz, seed);
+        c001 = lattice_wave(jx, jy, jz + 1, seed);     c101 = lattice_wave(jx + 1, jy, jz + 1, seed);
+        c011 = lattice_wave(jx, jy + 1, jz + 1, seed); c111 = lattice_wave(jx + 1, jy + 1, jz + 1, seed);
+    } else {
+        c000 = lattice(ix, iy, iz, seed);         c100 = lattice(ix + 1, iy, iz, seed);
+        c010 = lattice(ix, iy + 1, iz, seed);     c110 = lattice(ix + 1, iy + 1, iz, seed);
+        c001 = lattice(ix, iy, iz + 1, seed);     c101 = lattice(ix + 1, iy, iz + 1, seed);
+        c011 = lattice(ix, iy + 1, iz + 1, seed); c111 = lattice(ix + 1, iy + 1, iz + 1, seed);
+    }
     float x00 = lerpf_(c000, c100, wx), x10 = lerpf_(c010, c110, wx);
     float x01 = lerpf_(c001, c101, wx), x11 = lerpf_(c011, c111, wx);
     float y0 = lerpf_(x00, x10, wy), y1 = lerpf_(x01, x11, wy);
@@ -172,8 +193,9 @@ MP_DEV float vnoise(float x, float y, float z, uint32_t seed)
 MP_DEV float noise3(const DevParams& prm, float x, float y, float z)
 {
     float acc = 0.0f, amp = 1.0f, norm = 0.0f, freq = prm.disp_frequency;
+    const bool wave_cells = prm.noise_wave_cells != 0;
     for (int k = 0; k < prm.disp_octaves; ++k) {
-        acc = acc + amp * vnoise(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k);
+        acc = acc + amp * vnoise(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, wave_cells);
         norm = norm + amp;
         amp = amp * 0.5f;
         freq = freq * 2.0f;

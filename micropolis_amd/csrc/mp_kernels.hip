@@ -694,7 +694,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     constexpr int G = G1 * G1;
     constexpr int Q = N * N;
     constexpr int B = N / 8;
-    __shared__ float4 s_cp[20];
+    __shared__ float4 s_cpb[2][20];   // control points: current range / next range
     __shared__ float4 s_pos[G];
     __shared__ int2 s_pxl[G];
     __shared__ float s_dep[G];
@@ -716,22 +716,44 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     // chunked dynamic scheduling: one atomic per `chunk` ranges keeps the
     // shared counter off the critical path for million-range batches
     const uint32_t chunk = max(1u, min(64u, total / (gridDim.x * 16u)));
-    uint32_t r = 0, r_end = 0;
-    for (;;) {
-        if (r >= r_end) {
+    if (tid == 0) s_next = atomicAdd(&fc->raster_next, chunk);
+    __syncthreads();
+    uint32_t r = s_next;
+    uint32_t r_end = min(r + chunk, total);
+    if (r < total) {
+        const RangeRec rec0 = canon[r];
+        const DevObject& ob0 = objs[rec0.obj_depth & 0xffffffu];
+        const int ncp0 = ob0.type == MP_TYPE_GREGORY ? 20 : 16;
+        if (tid < ncp0) s_cpb[0][tid] = ob0.cps[(size_t)rec0.pid * ncp0 + tid];
+    }
+    __syncthreads();
+    int buf = 0;
+    while (r < total) {
+        // next range (the following chunk is claimed one range ahead) and its
+        // control points: loaded now, stored to the other LDS buffer after
+        // this range, so the load latency hides behind this range's work
+        uint32_t rn = r + 1, rn_end = r_end;
+        if (rn >= r_end) {
             if (tid == 0) s_next = atomicAdd(&fc->raster_next, chunk);
             __syncthreads();
-            r = s_next;
-            r_end = min(r + chunk, total);
-            __syncthreads();
+            rn = s_next;
+            rn_end = min(rn + chunk, total);
         }
-        if (r >= total) break;
+        float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool have_pf = false;
+        if (rn < total) {
+            const RangeRec nrec = canon[rn];
+            const DevObject& nob = objs[nrec.obj_depth & 0xffffffu];
+            const int nncp = nob.type == MP_TYPE_GREGORY ? 20 : 16;
+            if (tid < nncp) {
+                pf = nob.cps[(size_t)nrec.pid * nncp + tid];
+                have_pf = true;
+            }
+        }
+        const float4* s_cp = s_cpb[buf];
         const RangeRec rec = canon[r];
         const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
         const DevView& vw = views[ob.view];
-        const int ncp = ob.type == MP_TYPE_GREGORY ? 20 : 16;
-        if (tid < ncp) s_cp[tid] = ob.cps[(size_t)rec.pid * ncp + tid];
-        __syncthreads();
 
         // ---- dice (dice.cl:38-79) ----
         if (MODE != 2 || ob.type != MP_TYPE_BEZIER) {
@@ -940,7 +962,10 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
                 }
             }
         }
-        ++r;
+        if (have_pf) s_cpb[buf ^ 1][tid] = pf;
+        buf ^= 1;
+        r = rn;
+        r_end = rn_end;
         __syncthreads();
     }
     if (slow) atomicAdd(&fc->slow_blocks, slow);

@@ -74,6 +74,7 @@ struct DevParams {
     uint32_t disp_seed;
     int32_t disp_octaves;
     int32_t noise_cache;         // cache lattice values per range in LDS
+    int32_t noise_wave_cells;    // hash wave-uniform lattice cells once per wave
 };
 
 // Device-side frame counters, copied back once per frame.
