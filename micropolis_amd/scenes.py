@@ -156,8 +156,12 @@ def torus_patches(n=64, R=1.0, r=0.35):
     return np.array(pats, np.float32)
 
 
-TORUS_EYE = (0.55, -1.45, 0.42)
-TORUS_TARGET = (-0.25, 0.2, 0.0)
+# Camera of the 1080p torus frame: close to the tube, looking across the hole
+# at the far side of the ring.  30.4% of the 4096 patches lie in front of the
+# eye with a projected control-point box wider than 200 pixels, and the DRAW
+# ranges of the LOCAL split (limit 16) have depths 1..10.
+TORUS_EYE = (0.084, 1.388, -0.414)
+TORUS_TARGET = (-1.609, 0.233, -0.518)
 TORUS_UP = (0.0, 0.0, 1.0)
 
 
