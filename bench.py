@@ -49,6 +49,7 @@ def measure_batch(steps, use_graph):
         r.batch_draw(vi, o.handle, o.matrix, o.color)
     r.batch_finish()
     st = r.stats()
+    tuning = r.autotune(3) if use_graph else r.tuning()
     r.replay(1, use_graph=use_graph)
     batch_ms = r.replay(steps, use_graph=use_graph)
     r.close()
@@ -57,6 +58,7 @@ def measure_batch(steps, use_graph):
     return {"workload": w.name, "scenes": len(w.views), "patches": w.patch_count,
             "micropolygons": int(st["micropolygons"]), "plain_64_frames_ms": round(plain_ms, 4),
             "batched_ms": round(batch_ms, 4), "speedup": round(plain_ms / batch_ms, 3),
+            "batched_tuning": {k: tuning[k] for k in ("split_engine", "work_sharing", "raster_per_cu")},
             "same_micropolygons": int(mpolys) == int(st["micropolygons"])}
 
 
@@ -156,6 +158,9 @@ def main():
     r.finish()
     st = r.stats()
     use_graph = not args.no_graph
+    # pick the fastest scheduling of this frame (output unchanged)
+    tuning = r.autotune(5) if use_graph else r.tuning()
+    print("autotuned scheduling: %s" % tuning, file=sys.stderr, flush=True)
     print("first frame: %d ranges, %d micropolygons, %d coverage samples" % (
         st["diced_ranges"], st["micropolygons"], st["coverage_samples"]), file=sys.stderr, flush=True)
 
@@ -222,7 +227,8 @@ def main():
         "data": "synthetic (seeded torus patches, value-noise displacement)",
         "config": {"workload": w.name, "patches": P, "width": 1920, "height": 1080,
                    "reyes_patch_size": N, "bound_n_split_limit": 16, "method": "LOCAL",
-                   "graph": use_graph, "ranges": st["diced_ranges"], "micropolygons": mpolys,
+                   "graph": use_graph, "tuning": {k: tuning[k] for k in ("split_engine", "work_sharing", "raster_per_cu")},
+                   "ranges": st["diced_ranges"], "micropolygons": mpolys,
                    "coverage_samples": st["coverage_samples"]},
         "stages_ms": {k: round(x, 4) for k, x in stages.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

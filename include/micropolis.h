@@ -208,6 +208,25 @@ int mp_batch_read(mp_context* ctx, int32_t view, uint8_t* rgba8, float* depth,
  * measured with HIP events around the replays. ---- */
 int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* ms_per_frame);
 
+/* Scheduling of the device-driven frame (LOCAL / BALANCED).  None of these
+ * choices changes any output; they trade launch count, load balance and
+ * occupancy.  No reference counterpart (the reference fixes its schedule by
+ * bound_n_split_method and work-group sizes). */
+typedef struct mp_tuning {
+    int32_t split_engine;    /* 0: persistent LDS-stack split, one launch;
+                                1: one launch per split level + persistent tail */
+    int32_t work_sharing;    /* persistent split: deep stacks spill to idle groups */
+    int32_t raster_per_cu;   /* raster work-groups per CU (0 = occupancy limit) */
+    double  frame_ms;        /* mp_autotune: replay time of the chosen setting */
+} mp_tuning;
+int mp_get_tuning(mp_context* ctx, mp_tuning* tuning);
+/* Apply a setting; with a finished frame it is re-run once under it. */
+int mp_set_tuning(mp_context* ctx, const mp_tuning* tuning);
+/* Autotuner: replay the last finished LOCAL/BALANCED frame or batch under
+ * each candidate setting (split engine x work sharing x raster work-groups
+ * per CU), `replays` graph replays each, keep the fastest and report it. */
+int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen);
+
 /* Replay the last LOCAL/BALANCED frame n times with HIP events between its
  * stages on the library's stream; stage_ms[0..3] receive the mean device ms
  * of split, canonical ordering, raster (dice+shade+sample) and resolve. */

@@ -60,6 +60,11 @@ class mp_stats(C.Structure):
     ]
 
 
+class mp_tuning(C.Structure):
+    _fields_ = [("split_engine", C.c_int32), ("work_sharing", C.c_int32),
+                ("raster_per_cu", C.c_int32), ("frame_ms", C.c_double)]
+
+
 class mp_range_record(C.Structure):
     _fields_ = [("pid", C.c_int32), ("object", C.c_int32), ("depth", C.c_int32),
                 ("umin", C.c_float), ("vmin", C.c_float),
@@ -95,6 +100,9 @@ EXPORTS = [
     ("mp_backward", C.c_int, [_P, _P, _P, _P, _P]),
     ("mp_batch_read_device", C.c_int, [_P, C.c_int32, _P, _P]),
     ("mp_update_patches_device", C.c_int, [_P, C.c_uint64, _P, C.c_size_t]),
+    ("mp_get_tuning", C.c_int, [_P, C.POINTER(mp_tuning)]),
+    ("mp_set_tuning", C.c_int, [_P, C.POINTER(mp_tuning)]),
+    ("mp_autotune", C.c_int, [_P, C.c_int32, C.POINTER(mp_tuning)]),
     ("mp_cpu_bound_n_split", C.c_int, [C.POINTER(mp_config), _FP, C.c_size_t, C.c_int32, _FP,
                                        C.POINTER(mp_projection), C.POINTER(mp_range_record), C.c_int64,
                                        C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int32,

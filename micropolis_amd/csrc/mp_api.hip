@@ -122,7 +122,7 @@ struct mp_context {
     bool frame_valid = false;
     bool device_driven = false;  // the last frame can be replayed without host syncs
     hipGraphExec_t graph = nullptr;
-    bool work_sharing = true;    // spill queue between split groups
+    bool work_sharing = false;   // spill queue between split groups
     uint32_t grad_floats = 0;
     uint32_t* d_visible = nullptr; size_t visible_cap = 0;
     uint32_t* d_next = nullptr;
@@ -133,6 +133,8 @@ struct mp_context {
     uint32_t* d_level_cnt = nullptr; uint32_t* h_level_cnt = nullptr;
     int level_launches = 0;        // levels launched before the persistent tail
     int level_grid_size = 0;
+    int level_lanes = 9;           // lanes per range in the level kernels (9 or 1)
+    int level_per_cu = 1;          // level-kernel work-groups per CU (0: occupancy)
     int spill_at = 192;
     int linger = -1;             // groups that wait for spilled work (-1: one per CU)
     // CPU split method
@@ -338,13 +340,18 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     // entries.  Fewer resident groups that own their SIMD finish the deepest
     // split trees sooner than more groups sharing it (measured on the torus frame).
     ctx->stack_cap = 64 * (cfg->max_split_depth + 2);
-    // tuning knobs (defaults chosen by measurement; see the autotuner)
+    // tuning knobs (defaults chosen by measurement; mp_autotune picks per
+    // frame): BALANCED frames split fastest level by level (one shared queue
+    // serialises on its counter), LOCAL frames in the persistent kernel
+    ctx->split_engine = cfg->bound_n_split_method == MP_BNS_BALANCED ? 1 : 0;
     if (const char* e = getenv("MP_WORK_SHARING")) ctx->work_sharing = atoi(e) != 0;
     if (const char* e = getenv("MP_SPILL_AT")) ctx->spill_at = atoi(e);
     if (const char* e = getenv("MP_LINGER")) ctx->linger = atoi(e);
     if (const char* e = getenv("MP_STACK_CAP")) ctx->stack_cap = std::max(256, atoi(e));
     if (const char* e = getenv("MP_SPLIT_ENGINE")) ctx->split_engine = atoi(e);
     if (const char* e = getenv("MP_CPU_THREADS")) ctx->cpu_threads = atoi(e);
+    if (const char* e = getenv("MP_LEVEL_LANES")) ctx->level_lanes = atoi(e) == 1 ? 1 : 9;
+    if (const char* e = getenv("MP_LEVEL_PER_CU")) ctx->level_per_cu = atoi(e);
     ctx->level_launches = cfg->max_split_depth;
     if (const char* e = getenv("MP_RASTER_PER_CU")) ctx->raster_grid_size = ctx->num_cus * std::max(1, atoi(e));
     memset(&ctx->stats, 0, sizeof(ctx->stats));
@@ -759,6 +766,7 @@ static SplitQueues split_queues(mp_context* ctx)
     q.spill_at = ctx->spill_at;
     q.linger = ctx->linger >= 0 ? ctx->linger : ctx->num_cus;
     q.share = ctx->d_share;
+    q.pending = nullptr;
     return q;
 }
 
@@ -790,11 +798,12 @@ static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
     HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
     SplitQueues q = split_queues(ctx);
     if (ctx->total_roots && ctx->split_engine == 1) {
-        if (!ctx->level_grid_size) ctx->level_grid_size = level_grid(ctx->num_cus);
+        if (!ctx->level_grid_size)
+            ctx->level_grid_size = level_grid(ctx->num_cus, ctx->level_lanes, ctx->level_per_cu);
         HIPCHK(ctx, launch_split_levels(ctx->d_objs, (int)ctx->objects.size(), ctx->d_views, ctx->prm,
                                         ctx->total_roots, ctx->d_level_a, ctx->d_level_b, ctx->level_cap,
                                         ctx->d_level_cnt, ctx->level_launches, q, canon_buffers(ctx), ctx->d_fc,
-                                        ctx->level_grid_size, ctx->stream));
+                                        ctx->level_grid_size, ctx->level_lanes, ctx->stream));
     } else if (ctx->total_roots) {
         HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
                                        balanced, ctx->stream));
@@ -926,6 +935,20 @@ static int run_cpu_split(mp_context* ctx, uint32_t& draw_total, int64_t& max_pat
     return MP_OK;
 }
 
+// Level engine: launch only as many levels as the last frame needed (+1);
+// the persistent tail still finishes anything deeper.
+static int update_level_launches(mp_context* ctx, int64_t* max_patches)
+{
+    HIPCHK(ctx, hipMemcpy(ctx->h_level_cnt, ctx->d_level_cnt, 64 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    int deepest = 0;
+    for (int L = 0; L <= ctx->level_launches && L < 63; ++L) {
+        if (ctx->h_level_cnt[L]) deepest = L;
+        if (max_patches) *max_patches = std::max<int64_t>(*max_patches, ctx->h_level_cnt[L]);
+    }
+    ctx->level_launches = std::min(ctx->cfg.max_split_depth, deepest + 2);
+    return MP_OK;
+}
+
 static int execute_frame(mp_context* ctx)
 {
     int rc;
@@ -978,17 +1001,7 @@ static int execute_frame(mp_context* ctx)
             left -= ppp;
         } while (left > 0);
         max_patches = ctx->h_fc->max_stack;
-        if (ctx->split_engine == 1) {
-            // launch only as many levels as this scene needs (+1); the
-            // persistent tail still finishes anything deeper
-            HIPCHK(ctx, hipMemcpy(ctx->h_level_cnt, ctx->d_level_cnt, 64 * sizeof(uint32_t), hipMemcpyDeviceToHost));
-            int deepest = 0;
-            for (int L = 0; L <= ctx->level_launches && L < 63; ++L) {
-                if (ctx->h_level_cnt[L]) deepest = L;
-                max_patches = std::max<int64_t>(max_patches, ctx->h_level_cnt[L]);
-            }
-            ctx->level_launches = std::min(ctx->cfg.max_split_depth, deepest + 2);
-        }
+        if (ctx->split_engine == 1 && (rc = update_level_launches(ctx, &max_patches))) return rc;
     } else if (method == MP_BNS_CPU) {
         HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
         if ((rc = run_cpu_split(ctx, draw_total, max_patches))) return rc;
@@ -1253,6 +1266,84 @@ extern "C" int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* 
     HIPCHK(ctx, hipMemcpy(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost));
     if (ctx->h_fc->overflow) return fail(ctx, MP_E_STATE, "replay overflowed the draw list");
     if (ms_per_frame) *ms_per_frame = (double)ms / n;
+    return MP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// scheduling choices and the autotuner
+
+extern "C" int mp_get_tuning(mp_context* ctx, mp_tuning* t)
+{
+    if (!ctx || !t) return MP_E_INVALID;
+    t->split_engine = ctx->split_engine;
+    t->work_sharing = ctx->work_sharing ? 1 : 0;
+    t->raster_per_cu = ctx->raster_grid_size / std::max(1, ctx->num_cus);
+    t->frame_ms = 0.0;
+    return MP_OK;
+}
+
+// Applies a scheduling choice and, for a finished device-driven frame, runs
+// it once so that the level engine knows the frame's depth.
+static int apply_tuning(mp_context* ctx, const mp_tuning& t)
+{
+    ctx->split_engine = t.split_engine == 1 ? 1 : 0;
+    ctx->work_sharing = t.work_sharing != 0;
+    ctx->raster_grid_size = ctx->num_cus * std::max(1, t.raster_per_cu);
+    ctx->level_grid_size = 0;
+    if (ctx->split_engine == 1) ctx->level_launches = ctx->cfg.max_split_depth;
+    invalidate_graph(ctx);
+    if (!ctx->frame_valid || !ctx->device_driven) return MP_OK;
+    int rc;
+    if ((rc = ensure_queues(ctx))) return rc;
+    if ((rc = launch_device_frame(ctx))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->h_fc->overflow) return fail(ctx, MP_E_STATE, "frame overflowed under this tuning");
+    if (ctx->split_engine == 1 && (rc = update_level_launches(ctx, nullptr))) return rc;
+    return MP_OK;
+}
+
+extern "C" int mp_set_tuning(mp_context* ctx, const mp_tuning* t)
+{
+    if (!ctx || !t) return MP_E_INVALID;
+    if (t->split_engine < 0 || t->split_engine > 1 || t->raster_per_cu < 0 || t->raster_per_cu > 64)
+        return fail(ctx, MP_E_INVALID, "tuning out of range");
+    mp_tuning u = *t;
+    if (u.raster_per_cu == 0) u.raster_per_cu = raster_grid(ctx->cfg.reyes_patch_size, ctx->num_cus, ctx->prm) / ctx->num_cus;
+    return apply_tuning(ctx, u);
+}
+
+extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
+{
+    if (!ctx || replays < 1) return MP_E_INVALID;
+    if (!ctx->frame_valid || !ctx->device_driven)
+        return fail(ctx, MP_E_STATE, "autotune needs a finished LOCAL or BALANCED frame or batch");
+    const int occ = raster_grid(ctx->cfg.reyes_patch_size, ctx->num_cus, ctx->prm) / ctx->num_cus;
+    std::vector<mp_tuning> cand;
+    for (int per_cu : { occ, std::max(1, occ / 2) }) {
+        cand.push_back(mp_tuning{ 0, 0, per_cu, 0.0 });
+        cand.push_back(mp_tuning{ 0, 1, per_cu, 0.0 });
+        cand.push_back(mp_tuning{ 1, 0, per_cu, 0.0 });
+        if (occ / 2 < 1 || occ == 1) break;
+    }
+    // two rounds over the candidates, each candidate's best round counts
+    // (graph replays: the launch count of each schedule is part of its cost)
+    for (mp_tuning& c : cand) c.frame_ms = 1e30;
+    for (int round = 0; round < 2; ++round) {
+        for (mp_tuning& c : cand) {
+            if (apply_tuning(ctx, c) != MP_OK) continue;
+            double ms = 0.0;
+            if (mp_replay(ctx, replays, 1, &ms) != MP_OK) continue;
+            c.frame_ms = std::min(c.frame_ms, ms);
+        }
+    }
+    mp_tuning best{ -1, 0, 0, 1e30 };
+    for (const mp_tuning& c : cand)
+        if (c.frame_ms < best.frame_ms) best = c;
+    if (best.split_engine < 0) return fail(ctx, MP_E_STATE, "no tuning candidate ran");
+    int rc;
+    if ((rc = apply_tuning(ctx, best))) return rc;
+    if (chosen) *chosen = best;
     return MP_OK;
 }
 

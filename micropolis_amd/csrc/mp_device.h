@@ -308,31 +308,41 @@ MP_DEV void displaced_point(const float4* __restrict__ cp, int type, float u, fl
 }
 
 // ---- bound() (kernels/bound_n_split.h:43-100) with RES = 3 ----
-MP_DEV int bound3(const float4* __restrict__ cp, int type, int depth, float umin, float vmin,
-                  float umax, float vmax, const float* __restrict__ mv, const DevView& P,
-                  const DevParams& prm, float split_limit)
+// One of bound()'s 3x3 samples: eye-space point and screen position
+// (bound_n_split.h:52-68).
+template <int NCP>
+MP_DEV void bound_sample(const float4 (&C)[NCP], int iu, int iv, float umin, float vmin, float umax,
+                         float vmax, const float* __restrict__ mv, const DevView& P, float e[3],
+                         float& sx, float& sy)
 {
-    if (depth >= prm.max_split_depth - 1) return MP_CULL;
-    float px[3][3], py[3][3];
+    const float tu = (float)iu * 0.5f, tv = (float)iv * 0.5f;
+    const float u = umin + (umax - umin) * tu;
+    const float v = vmin + (vmax - vmin) * tv;
+    float p[4], ee[4], q[4];
+    if (NCP == 20) eval_gregory(C, u, v, p);
+    else           eval_bezier(C, u, v, p);
+    mat4_mul(mv, p, ee);
+    mat4_mul(P.proj, ee, q);
+    const float rx = q[0] / q[3], ry = q[1] / q[3];
+    sx = P.screen[0] * rx + P.screen[2] * ry;
+    sy = P.screen[1] * rx + P.screen[3] * ry;
+    e[0] = ee[0]; e[1] = ee[1]; e[2] = ee[2];
+}
+
+// bound()'s decision from its nine samples, sample k = iu * 3 + iv
+// (bound_n_split.h:10-31, 70-100): box in sample order, frustum cull,
+// eye split, polyline lengths.
+MP_DEV int bound_decide(const float (&ex)[9], const float (&ey)[9], const float (&ez)[9],
+                        const float (&sx)[9], const float (&sy)[9], float umin, float vmin,
+                        float umax, float vmax, const DevView& P, const DevParams& prm,
+                        float split_limit)
+{
     float bminx = INFINITY, bminy = INFINITY, bminz = INFINITY;
     float bmaxx = -INFINITY, bmaxy = -INFINITY, bmaxz = -INFINITY;
 #pragma unroll
-    for (int iu = 0; iu < 3; ++iu) {
-#pragma unroll
-        for (int iv = 0; iv < 3; ++iv) {
-            float tu = (float)iu * 0.5f, tv = (float)iv * 0.5f;
-            float u = umin + (umax - umin) * tu;
-            float v = vmin + (vmax - vmin) * tv;
-            float p[4], e[4], q[4];
-            eval_patch(cp, type, u, v, p);
-            mat4_mul(mv, p, e);
-            bminx = cl_min(bminx, e[0]); bminy = cl_min(bminy, e[1]); bminz = cl_min(bminz, e[2]);
-            bmaxx = cl_max(bmaxx, e[0]); bmaxy = cl_max(bmaxy, e[1]); bmaxz = cl_max(bmaxz, e[2]);
-            mat4_mul(P.proj, e, q);
-            float rx = q[0] / q[3], ry = q[1] / q[3];
-            px[iv][iu] = P.screen[0] * rx + P.screen[2] * ry;
-            py[iv][iu] = P.screen[1] * rx + P.screen[3] * ry;
-        }
+    for (int k = 0; k < 9; ++k) {
+        bminx = cl_min(bminx, ex[k]); bminy = cl_min(bminy, ey[k]); bminz = cl_min(bminz, ez[k]);
+        bmaxx = cl_max(bmaxx, ex[k]); bmaxy = cl_max(bmaxy, ey[k]); bmaxz = cl_max(bmaxz, ez[k]);
     }
     // outside_frustum (bound_n_split.h:10-31)
     {
@@ -352,17 +362,18 @@ MP_DEV int bound3(const float4* __restrict__ cp, int type, int depth, float umin
     }
     if (bminz < P.eps && bmaxz > P.near_)
         return ((umax - umin) < (vmax - vmin)) ? MP_VSPLIT : MP_HSPLIT;
+    // px[iv][iu] = sx[iu * 3 + iv]
     float hlen = 0.0f, vlen = 0.0f;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         float h = 0.0f, v = 0.0f;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            float dx = px[i][j] - px[i][j + 1];
-            float dy = py[i][j] - py[i][j + 1];
+            float dx = sx[j * 3 + i] - sx[(j + 1) * 3 + i];
+            float dy = sy[j * 3 + i] - sy[(j + 1) * 3 + i];
             h = h + sqrtf(dx * dx + dy * dy);
-            dx = px[j][i] - px[j + 1][i];
-            dy = py[j][i] - py[j + 1][i];
+            dx = sx[i * 3 + j] - sx[i * 3 + j + 1];
+            dy = sy[i * 3 + j] - sy[i * 3 + j + 1];
             v = v + sqrtf(dx * dx + dy * dy);
         }
         hlen = cl_max(h, hlen);
@@ -370,6 +381,44 @@ MP_DEV int bound3(const float4* __restrict__ cp, int type, int depth, float umin
     }
     if (hlen <= split_limit && vlen <= split_limit) return MP_DRAW;
     return (hlen > vlen) ? MP_VSPLIT : MP_HSPLIT;
+}
+
+// The control points are loaded into registers once per range (NCP = 16
+// Bezier, 20 Gregory): the nine evaluations then cost one memory round trip.
+template <int NCP>
+MP_DEV int bound3_regs(const float4 (&C)[NCP], float umin, float vmin, float umax, float vmax,
+                       const float* __restrict__ mv, const DevView& P, const DevParams& prm,
+                       float split_limit)
+{
+    float ex[9], ey[9], ez[9], sx[9], sy[9];
+#pragma unroll
+    for (int iu = 0; iu < 3; ++iu) {
+#pragma unroll
+        for (int iv = 0; iv < 3; ++iv) {
+            float e[3];
+            const int k = iu * 3 + iv;
+            bound_sample<NCP>(C, iu, iv, umin, vmin, umax, vmax, mv, P, e, sx[k], sy[k]);
+            ex[k] = e[0]; ey[k] = e[1]; ez[k] = e[2];
+        }
+    }
+    return bound_decide(ex, ey, ez, sx, sy, umin, vmin, umax, vmax, P, prm, split_limit);
+}
+
+MP_DEV int bound3(const float4* __restrict__ cp, int type, int depth, float umin, float vmin,
+                  float umax, float vmax, const float* __restrict__ mv, const DevView& P,
+                  const DevParams& prm, float split_limit)
+{
+    if (depth >= prm.max_split_depth - 1) return MP_CULL;
+    if (type == MP_TYPE_GREGORY) {
+        float4 C[20];
+#pragma unroll
+        for (int k = 0; k < 20; ++k) C[k] = cp[k];
+        return bound3_regs<20>(C, umin, vmin, umax, vmax, mv, P, prm, split_limit);
+    }
+    float4 C[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) C[k] = cp[k];
+    return bound3_regs<16>(C, umin, vmin, umax, vmax, mv, P, prm, split_limit);
 }
 
 // Children of a split range (bound_n_split_breadthfirst.cl:99-113)

@@ -29,6 +29,7 @@ struct SplitQueues {
     int32_t spill_at;        // spill 64 ranges when the stack exceeds this
     int32_t linger;          // idle groups wait (bounded) for spilled chunks
     ShareState* share;
+    const uint32_t* pending; // if set and zero on entry, there is no work: exit
 };
 
 struct CanonBuffers {
@@ -87,11 +88,11 @@ hipError_t launch_adjoint(int patch_size, const RangeRec* canon, const DevObject
                           uint32_t* visible, uint32_t* next, float* grad_cps, float* grad_color,
                           float* grad_amp, int grid, hipStream_t s);
 int adjoint_grid(int patch_size, int num_cus);
-int level_grid(int num_cus);
+int level_grid(int num_cus, int lanes_per_range, int per_cu);
 hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView* views, const DevParams& prm,
                                uint32_t total_roots, RangeRec* list_a, RangeRec* list_b, uint32_t list_cap,
                                uint32_t* level_cnt, int n_levels, const SplitQueues& q,
-                               const CanonBuffers& cb, FrameCounters* fc, int grid, hipStream_t s);
+                               const CanonBuffers& cb, FrameCounters* fc, int grid, int lanes_per_range, hipStream_t s);
 hipError_t launch_xyz_to_cps(const float* xyz, float4* cps, size_t n, hipStream_t s);
 int split_persistent_occupancy(size_t lds_bytes);
 

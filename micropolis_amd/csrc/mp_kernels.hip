@@ -198,6 +198,7 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
     extern __shared__ RangeRec ring[];
     const int lane = threadIdx.x;
     const int wid = blockIdx.x;
+    if (q.pending && *q.pending == 0u) return;   // nothing left by the level launches
     const int cap = q.stack_cap;
     RangeRec* myq = q.queue + (int64_t)wid * q.stride;
     int base = 0, sh = 0;
@@ -1544,6 +1545,104 @@ __global__ __launch_bounds__(256) void k_level(const DevObject* __restrict__ obj
     if (lane == 0 && nb) atomicAdd(&fc->bounds, (unsigned long long)nb);
 }
 
+// Same level step with nine lanes per range: lane 9g+s of a wave evaluates
+// sample s = iu*3+iv of the wave's range g (7 ranges per wave), and the
+// group's first lane gathers the nine samples and takes bound()'s decision
+// in the sequential sample order.  One range costs one patch evaluation of
+// latency instead of nine.
+__global__ __launch_bounds__(256) void k_level9(const DevObject* __restrict__ objs,
+                                                const DevView* __restrict__ views, DevParams prm,
+                                                const RangeRec* __restrict__ cur, RangeRec* __restrict__ nxt,
+                                                uint32_t* level_cnt, int L, uint32_t nxt_cap,
+                                                CanonBuffers cb, FrameCounters* fc)
+{
+    const uint32_t n = min(level_cnt[L], nxt_cap);
+    const int lane = threadIdx.x & 63;
+    const int grp = lane / 9, smp = lane - 9 * grp;
+    const int iu = smp / 3, iv = smp - 3 * iu;
+    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t n_waves = gridDim.x * 4u;
+    uint32_t nb = 0;
+    for (uint32_t base = wave * 7u; base < n; base += n_waves * 7u) {
+        const uint32_t i = base + (uint32_t)grp;
+        const bool valid = grp < 7 && i < n;
+        RangeRec rec;
+        rec.pid = 0; rec.obj_depth = 0; rec.umin = rec.vmin = rec.umax = rec.vmax = 0.0f;
+        float e[3] = { 0.0f, 0.0f, 0.0f };
+        float sx = 0.0f, sy = 0.0f;
+        bool live = false;
+        if (valid) {
+            rec = cur[i];
+            const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
+            live = (int)(rec.obj_depth >> 24) < prm.max_split_depth - 1;
+            if (live) {
+                const DevView& P = views[ob.view];
+                if (ob.type == MP_TYPE_GREGORY) {
+                    const float4* cp = ob.cps + (size_t)rec.pid * 20;
+                    float4 C[20];
+#pragma unroll
+                    for (int k = 0; k < 20; ++k) C[k] = cp[k];
+                    bound_sample<20>(C, iu, iv, rec.umin, rec.vmin, rec.umax, rec.vmax, ob.mv, P, e, sx, sy);
+                } else {
+                    const float4* cp = ob.cps + (size_t)rec.pid * 16;
+                    float4 C[16];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) C[k] = cp[k];
+                    bound_sample<16>(C, iu, iv, rec.umin, rec.vmin, rec.umax, rec.vmax, ob.mv, P, e, sx, sy);
+                }
+            }
+        }
+        float gx[9], gy[9], gz[9], gsx[9], gsy[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int src = min(grp * 9 + k, 63);
+            gx[k] = __shfl(e[0], src);
+            gy[k] = __shfl(e[1], src);
+            gz[k] = __shfl(e[2], src);
+            gsx[k] = __shfl(sx, src);
+            gsy[k] = __shfl(sy, src);
+        }
+        const bool leader = valid && smp == 0;
+        int flags = 0;
+        if (leader && live) {
+            const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
+            flags = bound_decide(gx, gy, gz, gsx, gsy, rec.umin, rec.vmin, rec.umax, rec.vmax, views[ob.view],
+                                 prm, ob.split_limit);
+        }
+        const uint64_t sb = __ballot(flags & 2);
+        const int ns = __popcll(sb);
+        if (ns) {
+            uint32_t nbase = 0;
+            if (lane == 0) nbase = atomicAdd(&level_cnt[L + 1], 2u * (uint32_t)ns);
+            nbase = __shfl(nbase, 0);
+            if (flags & 2) {
+                const uint32_t k = nbase + 2u * lanes_below(sb);
+                if (k + 1 < nxt_cap) {
+                    RangeRec c0, c1;
+                    split_children(rec, flags, c0, c1);
+                    nxt[k] = c0;
+                    nxt[k + 1] = c1;
+                }
+            }
+            if (nbase + 2u * (uint32_t)ns > nxt_cap && lane == 0) atomicOr(&fc->overflow, 8u);
+        }
+        const uint64_t db = __ballot(flags & 1);
+        const int nd = __popcll(db);
+        if (nd) {
+            uint32_t dbase = 0;
+            if (lane == 0) dbase = atomicAdd(&fc->draw_count, (uint32_t)nd);
+            dbase = __shfl(dbase, 0);
+            const uint32_t idx = dbase + lanes_below(db);
+            const bool put = (flags & 1) && idx < cb.cap;
+            if (put) cb.draw[idx] = rec;
+            wave_count_add(cb.counts, put ? gpid_of(objs, rec) : 0u, put);
+            if (dbase + (uint32_t)nd > cb.cap && lane == 0) atomicOr(&fc->overflow, 1u);
+        }
+        nb += (uint32_t)__popcll(__ballot(leader));
+    }
+    if (lane == 0 && nb) atomicAdd(&fc->bounds, (unsigned long long)nb);
+}
+
 // Leftover ranges after the last launched level go to the persistent
 // engine's private queues (round-robin).
 __global__ void k_level_to_queues(const RangeRec* __restrict__ list, const uint32_t* level_cnt, int L,
@@ -1559,18 +1658,24 @@ __global__ void k_level_to_queues(const RangeRec* __restrict__ list, const uint3
     }
 }
 
-int level_grid(int num_cus)
+int level_grid(int num_cus, int lanes_per_range, int per_cu)
 {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_level, 256, 0) != hipSuccess || blocks < 1)
-        blocks = 1;
+    const hipError_t e = lanes_per_range == 9
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_level9, 256, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_level, 256, 0);
+    if (e != hipSuccess || blocks < 1) blocks = 1;
+    // few work-groups per CU: a level launch costs its dispatch, and levels
+    // are small (thousands of ranges)
+    if (per_cu > 0 && per_cu < blocks) blocks = per_cu;
     return blocks * num_cus;
 }
 
 hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView* views, const DevParams& prm,
                                uint32_t total_roots, RangeRec* list_a, RangeRec* list_b, uint32_t list_cap,
                                uint32_t* level_cnt, int n_levels, const SplitQueues& q,
-                               const CanonBuffers& cb, FrameCounters* fc, int grid, hipStream_t s)
+                               const CanonBuffers& cb, FrameCounters* fc, int grid, int lanes_per_range,
+                               hipStream_t s)
 {
     const uint32_t n0 = total_roots > (uint32_t)(n_levels + 1) ? total_roots : (uint32_t)(n_levels + 1);
     hipLaunchKernelGGL(k_level_roots, dim3((n0 + 255) / 256), dim3(256), 0, s, objs, n_objs, total_roots,
@@ -1580,8 +1685,12 @@ hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView*
     for (int L = 0; L < n_levels; ++L) {
         const RangeRec* cur = (L & 1) ? list_b : list_a;
         RangeRec* nxt = (L & 1) ? list_a : list_b;
-        hipLaunchKernelGGL(k_level, dim3(grid), dim3(256), 0, s, objs, views, prm, cur, nxt, level_cnt, L,
-                           list_cap, cb, fc);
+        if (lanes_per_range == 9)
+            hipLaunchKernelGGL(k_level9, dim3(grid), dim3(256), 0, s, objs, views, prm, cur, nxt, level_cnt, L,
+                               list_cap, cb, fc);
+        else
+            hipLaunchKernelGGL(k_level, dim3(grid), dim3(256), 0, s, objs, views, prm, cur, nxt, level_cnt, L,
+                               list_cap, cb, fc);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -1592,7 +1701,9 @@ hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView*
                        left, level_cnt, n_levels, q, list_cap);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_split_persistent(objs, views, prm, q, cb, fc, 0, s);
+    SplitQueues qt = q;
+    qt.pending = level_cnt + n_levels;
+    return launch_split_persistent(objs, views, prm, qt, cb, fc, 0, s);
 }
 
 }  // namespace mp

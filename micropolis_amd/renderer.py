@@ -226,6 +226,24 @@ class Renderer:
         self._check(self._lib.mp_replay_stages(self._ctx, int(n), ms))
         return dict(zip(("split", "order", "raster", "resolve"), list(ms)))
 
+    def tuning(self):
+        t = _lib.mp_tuning()
+        self._check(self._lib.mp_get_tuning(self._ctx, C.byref(t)))
+        return {"split_engine": t.split_engine, "work_sharing": t.work_sharing,
+                "raster_per_cu": t.raster_per_cu}
+
+    def set_tuning(self, split_engine=0, work_sharing=0, raster_per_cu=0):
+        t = _lib.mp_tuning(int(split_engine), int(work_sharing), int(raster_per_cu), 0.0)
+        self._check(self._lib.mp_set_tuning(self._ctx, C.byref(t)))
+
+    def autotune(self, replays=5):
+        """Pick the fastest scheduling of the last LOCAL/BALANCED frame or
+        batch (the output does not change)."""
+        t = _lib.mp_tuning()
+        self._check(self._lib.mp_autotune(self._ctx, int(replays), C.byref(t)))
+        return {"split_engine": t.split_engine, "work_sharing": t.work_sharing,
+                "raster_per_cu": t.raster_per_cu, "frame_ms": t.frame_ms}
+
     def replay(self, n, use_graph=False):
         ms = C.c_double()
         self._check(self._lib.mp_replay(self._ctx, int(n), int(bool(use_graph)), C.byref(ms)))

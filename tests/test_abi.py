@@ -29,13 +29,15 @@ def test_struct_sizes_match_header_layout(tmp_path):
     import subprocess
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include "micropolis.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(mp_config), '
-                   'sizeof(mp_projection), sizeof(mp_range_record), sizeof(mp_stats));return 0;}\n')
+                   'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(mp_config), '
+                   'sizeof(mp_projection), sizeof(mp_range_record), sizeof(mp_stats), '
+                   'sizeof(mp_tuning));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     assert sizes == [C.sizeof(_lib.mp_config), C.sizeof(_lib.mp_projection),
-                     C.sizeof(_lib.mp_range_record), C.sizeof(_lib.mp_stats)]
+                     C.sizeof(_lib.mp_range_record), C.sizeof(_lib.mp_stats),
+                     C.sizeof(_lib.mp_tuning)]
 
 
 def test_defaults_are_reyes_options():

@@ -72,3 +72,37 @@ def test_cpu_method_gregory_and_replay(oracle):
     again = r.read_framebuffer()
     assert np.array_equal(again["ids"], got["ids"])
     r.close()
+
+
+def _frame_of(r):
+    out = r.read_framebuffer()
+    out["ranges"] = r.ranges()
+    return out
+
+
+def test_every_scheduling_gives_the_same_frame(oracle):
+    """Split engine and work sharing change only the schedule (BALANCED)."""
+    w = scenes.c4_gregory(planes=8, quads=128, width=960, height=540)
+    print("case c4_gregory small BALANCED tunings", flush=True)
+    ref = oracle.render(w)[0]
+    r, got = render_plain(w)
+    assert_frame_equal(got, ref, "c4-small/default")
+    for engine in (0, 1):
+        for sharing in (0, 1):
+            r.set_tuning(split_engine=engine, work_sharing=sharing)
+            r.replay(2, use_graph=True)
+            assert_frame_equal(_frame_of(r), ref, "c4-small/engine%d-sharing%d" % (engine, sharing))
+    r.close()
+
+
+def test_autotune_keeps_the_frame(oracle):
+    w = scenes.c2_torus(width=640, height=360)
+    print("case c2_torus 640x360 autotune", flush=True)
+    ref = oracle.render(w)[0]
+    r, got = render_plain(w)
+    t = r.autotune(2)
+    assert t["split_engine"] in (0, 1) and t["frame_ms"] > 0.0
+    assert r.tuning()["split_engine"] == t["split_engine"]
+    r.replay(2, use_graph=True)
+    assert_frame_equal(_frame_of(r), ref, "c2-small/autotuned")
+    r.close()
