@@ -14,6 +14,35 @@
 
 #define MP_DEV __device__ __forceinline__
 
+// Global-address-space view of a device pointer held in a table (object,
+// view or queue records).  Through it the compiler emits global_* memory
+// instructions instead of flat_* ones; flat instructions also count against
+// the LDS/scalar wait counter, so every later wait for an LDS result would
+// wait for them as well.
+#define MP_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+MP_DEV MP_GLOBAL T* gptr(T* p) { return (MP_GLOBAL T*)p; }
+
+typedef float mp_f4v __attribute__((ext_vector_type(4)));
+// 16-byte global load / store of a float4 held behind a table pointer.
+MP_DEV float4 ldg4(const float4* p)
+{
+    const mp_f4v v = *(const MP_GLOBAL mp_f4v*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+MP_DEV void stg4(float4* p, const float4& v)
+{
+    mp_f4v w;
+    w.x = v.x; w.y = v.y; w.z = v.z; w.w = v.w;
+    *(MP_GLOBAL mp_f4v*)p = w;
+}
+
+// Visibility update: 64-bit atomic minimum, relaxed, device scope, no return.
+MP_DEV void key_min(unsigned long long* p, unsigned long long v)
+{
+    __hip_atomic_fetch_min(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 namespace mp {
 
 // OpenCL min(x,y): y if y < x else x.  OpenCL max(x,y): y if x < y else x.
@@ -412,12 +441,12 @@ MP_DEV int bound3(const float4* __restrict__ cp, int type, int depth, float umin
     if (type == MP_TYPE_GREGORY) {
         float4 C[20];
 #pragma unroll
-        for (int k = 0; k < 20; ++k) C[k] = cp[k];
+        for (int k = 0; k < 20; ++k) C[k] = ldg4(cp + k);
         return bound3_regs<20>(C, umin, vmin, umax, vmax, mv, P, prm, split_limit);
     }
     float4 C[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) C[k] = cp[k];
+    for (int k = 0; k < 16; ++k) C[k] = ldg4(cp + k);
     return bound3_regs<16>(C, umin, vmin, umax, vmax, mv, P, prm, split_limit);
 }
 

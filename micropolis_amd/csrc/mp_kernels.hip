@@ -725,7 +725,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
         const RangeRec rec0 = canon[r];
         const DevObject& ob0 = objs[rec0.obj_depth & 0xffffffu];
         const int ncp0 = ob0.type == MP_TYPE_GREGORY ? 20 : 16;
-        if (tid < ncp0) s_cpb[0][tid] = ob0.cps[(size_t)rec0.pid * ncp0 + tid];
+        if (tid < ncp0) s_cpb[0][tid] = ldg4(ob0.cps + (size_t)rec0.pid * ncp0 + tid);
     }
     __syncthreads();
     int buf = 0;
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
             const DevObject& nob = objs[nrec.obj_depth & 0xffffffu];
             const int nncp = nob.type == MP_TYPE_GREGORY ? 20 : 16;
             if (tid < nncp) {
-                pf = nob.cps[(size_t)nrec.pid * nncp + tid];
+                pf = ldg4(nob.cps + (size_t)nrec.pid * nncp + tid);
                 have_pf = true;
             }
         }
@@ -929,7 +929,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
                             const bool in2 = inside_tri(x0.x, x0.y, x3.x, x3.y, x2.x, x2.y, d0, d3, d2, tx, ty, dd);
                             if (in1 || in2) {
                                 const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
-                                atomicMin(&keys[fb_index(X, Y, gw)], key);
+                                key_min(&keys[fb_index(X, Y, gw)], key);
                                 ++natom;
                             }
                         }
@@ -954,7 +954,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
                                 const bool in2 = inside_tri(x0.x, x0.y, x3.x, x3.y, x2.x, x2.y, d0, d3, d2, tpx, tpy, dd);
                                 if (in1 || in2) {
                                     const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
-                                    atomicMin(&keys[fb_index(tx * 8 + x, ty * 8 + y, gw)], key);
+                                    key_min(&keys[fb_index(tx * 8 + x, ty * 8 + y, gw)], key);
                                     ++natom;
                                 }
                             }
@@ -1054,8 +1054,8 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
     const uint32_t n = (uint32_t)vw.grid_w * (uint32_t)vw.grid_h * 64u;
     const float ac[4] = { 0.015f, 0.015f, 0.015f, 1.0f };
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const unsigned long long key = vw.keys[i];
-        vw.keys[i] = MP_EMPTY_KEY;
+        const unsigned long long key = gptr(vw.keys)[i];
+        gptr(vw.keys)[i] = MP_EMPTY_KEY;
         const uint32_t t = i >> 6, l = i & 63;
         const int X = (int)(t % (uint32_t)vw.grid_w) * 8 + (int)(l & 7);
         const int Y = (int)(t / (uint32_t)vw.grid_w) * 8 + (int)(l >> 3);
@@ -1078,10 +1078,11 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
         }
         uchar4 px;
         px.x = to_srgb8(c[0]); px.y = to_srgb8(c[1]); px.z = to_srgb8(c[2]); px.w = 255;
-        reinterpret_cast<uchar4*>(vw.rgba8)[o] = px;
-        vw.depth[o] = d;
-        vw.ids[o] = id;
-        if (write_rgba32) reinterpret_cast<float4*>(vw.rgba32)[o] = make_float4(c[0], c[1], c[2], c[3]);
+        *gptr(reinterpret_cast<uint32_t*>(vw.rgba8) + o) =
+            (uint32_t)px.x | ((uint32_t)px.y << 8) | ((uint32_t)px.z << 16) | ((uint32_t)px.w << 24);
+        gptr(vw.depth)[o] = d;
+        gptr(vw.ids)[o] = id;
+        if (write_rgba32) stg4(reinterpret_cast<float4*>(vw.rgba32) + o, make_float4(c[0], c[1], c[2], c[3]));
     }
 }
 
@@ -1159,7 +1160,7 @@ __global__ __launch_bounds__(256) void k_adj_mark(const DevView* __restrict__ vi
     const DevView& vw = views[blockIdx.y];
     const uint32_t n = (uint32_t)vw.width * (uint32_t)vw.height;
     for (uint32_t o = blockIdx.x * 256 + threadIdx.x; o < n; o += gridDim.x * 256) {
-        const uint32_t id = vw.ids[o];
+        const uint32_t id = gptr(vw.ids)[o];
         if (id != 0xFFFFFFFFu) visible[vw.range_base + id / (uint32_t)Q] = 1u;
     }
 }
@@ -1259,7 +1260,7 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
         const RangeRec rec = canon[r];
         const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
         const DevView& vw = views[ob.view];
-        if (tid < 16) s_cp[tid] = ob.cps[(size_t)rec.pid * 16 + tid];
+        if (tid < 16) s_cp[tid] = ldg4(ob.cps + (size_t)rec.pid * 16 + tid);
         __syncthreads();
 
         // forward state of the grid points owned by this thread
@@ -1326,7 +1327,7 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
                 for (int Y = Y0; Y <= Y1; ++Y)
                     for (int X = X0; X <= X1; ++X) {
                         const size_t o = (size_t)Y * vw.width + X;
-                        if (vw.ids[o] != myid) continue;
+                        if (gptr(vw.ids)[o] != myid) continue;
                         const float* dl = dl_dcolor + ((size_t)vw.pixel_offset + o) * 4;
                         gs += dl[0] * ob.color[0] + dl[1] * ob.color[1] + dl[2] * ob.color[2];
                         ga0 += dl[0]; ga1 += dl[1]; ga2 += dl[2];
@@ -1581,13 +1582,13 @@ __global__ __launch_bounds__(256) void k_level9(const DevObject* __restrict__ ob
                     const float4* cp = ob.cps + (size_t)rec.pid * 20;
                     float4 C[20];
 #pragma unroll
-                    for (int k = 0; k < 20; ++k) C[k] = cp[k];
+                    for (int k = 0; k < 20; ++k) C[k] = ldg4(cp + k);
                     bound_sample<20>(C, iu, iv, rec.umin, rec.vmin, rec.umax, rec.vmax, ob.mv, P, e, sx, sy);
                 } else {
                     const float4* cp = ob.cps + (size_t)rec.pid * 16;
                     float4 C[16];
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) C[k] = cp[k];
+                    for (int k = 0; k < 16; ++k) C[k] = ldg4(cp + k);
                     bound_sample<16>(C, iu, iv, rec.umin, rec.vmin, rec.umax, rec.vmax, ob.mv, P, e, sx, sy);
                 }
             }
