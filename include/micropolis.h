@@ -217,14 +217,18 @@ typedef struct mp_tuning {
                                 1: one launch per split level + persistent tail */
     int32_t work_sharing;    /* persistent split: deep stacks spill to idle groups */
     int32_t raster_per_cu;   /* raster work-groups per CU (0 = occupancy limit) */
+    int32_t raster_chunk;    /* raster range schedule: 0 static stride (no
+                                atomics), -1 dynamic chunks sized from the
+                                range count, > 0 dynamic chunks of that size */
     double  frame_ms;        /* mp_autotune: replay time of the chosen setting */
 } mp_tuning;
 int mp_get_tuning(mp_context* ctx, mp_tuning* tuning);
 /* Apply a setting; with a finished frame it is re-run once under it. */
 int mp_set_tuning(mp_context* ctx, const mp_tuning* tuning);
 /* Autotuner: replay the last finished LOCAL/BALANCED frame or batch under
- * each candidate setting (split engine x work sharing x raster work-groups
- * per CU), `replays` graph replays each, keep the fastest and report it. */
+ * candidate settings, `replays` graph replays each (best of two rounds):
+ * first the raster schedule (work-groups per CU x static / dynamic ranges),
+ * then the split (engine x work sharing); keep the fastest and report it. */
 int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen);
 
 /* Replay the last LOCAL/BALANCED frame n times with HIP events between its

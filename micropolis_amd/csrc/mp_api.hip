@@ -334,6 +334,8 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     ctx->prm.noise_cache = 0;
     if (const char* e = getenv("MP_NOISE_CACHE")) ctx->prm.noise_cache = atoi(e);
     ctx->prm.noise_wave_cells = 1;
+    ctx->prm.raster_chunk = 0;   // static range stride (see mp_tuning)
+    if (const char* e = getenv("MP_RASTER_CHUNK")) ctx->prm.raster_chunk = atoi(e);
     if (const char* e = getenv("MP_NOISE_WAVE_CELLS")) ctx->prm.noise_wave_cells = atoi(e);
     ctx->raster_grid_size = raster_grid(cfg->reyes_patch_size, ctx->num_cus, ctx->prm);
     // LDS ring stack of the persistent split, depth-bounded: 64 * (max_split_depth + 2)
@@ -1278,6 +1280,7 @@ extern "C" int mp_get_tuning(mp_context* ctx, mp_tuning* t)
     t->split_engine = ctx->split_engine;
     t->work_sharing = ctx->work_sharing ? 1 : 0;
     t->raster_per_cu = ctx->raster_grid_size / std::max(1, ctx->num_cus);
+    t->raster_chunk = ctx->prm.raster_chunk;
     t->frame_ms = 0.0;
     return MP_OK;
 }
@@ -1289,6 +1292,7 @@ static int apply_tuning(mp_context* ctx, const mp_tuning& t)
     ctx->split_engine = t.split_engine == 1 ? 1 : 0;
     ctx->work_sharing = t.work_sharing != 0;
     ctx->raster_grid_size = ctx->num_cus * std::max(1, t.raster_per_cu);
+    ctx->prm.raster_chunk = t.raster_chunk;
     ctx->level_grid_size = 0;
     if (ctx->split_engine == 1) ctx->level_launches = ctx->cfg.max_split_depth;
     invalidate_graph(ctx);
@@ -1306,28 +1310,18 @@ static int apply_tuning(mp_context* ctx, const mp_tuning& t)
 extern "C" int mp_set_tuning(mp_context* ctx, const mp_tuning* t)
 {
     if (!ctx || !t) return MP_E_INVALID;
-    if (t->split_engine < 0 || t->split_engine > 1 || t->raster_per_cu < 0 || t->raster_per_cu > 64)
+    if (t->split_engine < 0 || t->split_engine > 1 || t->raster_per_cu < 0 || t->raster_per_cu > 64 ||
+        t->raster_chunk < -1 || t->raster_chunk > 4096)
         return fail(ctx, MP_E_INVALID, "tuning out of range");
     mp_tuning u = *t;
     if (u.raster_per_cu == 0) u.raster_per_cu = raster_grid(ctx->cfg.reyes_patch_size, ctx->num_cus, ctx->prm) / ctx->num_cus;
     return apply_tuning(ctx, u);
 }
 
-extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
+// Best of two rounds of `replays` graph replays per candidate; returns the
+// index of the fastest candidate that ran, or -1.
+static int pick_fastest(mp_context* ctx, std::vector<mp_tuning>& cand, int replays)
 {
-    if (!ctx || replays < 1) return MP_E_INVALID;
-    if (!ctx->frame_valid || !ctx->device_driven)
-        return fail(ctx, MP_E_STATE, "autotune needs a finished LOCAL or BALANCED frame or batch");
-    const int occ = raster_grid(ctx->cfg.reyes_patch_size, ctx->num_cus, ctx->prm) / ctx->num_cus;
-    std::vector<mp_tuning> cand;
-    for (int per_cu : { occ, std::max(1, occ / 2) }) {
-        cand.push_back(mp_tuning{ 0, 0, per_cu, 0.0 });
-        cand.push_back(mp_tuning{ 0, 1, per_cu, 0.0 });
-        cand.push_back(mp_tuning{ 1, 0, per_cu, 0.0 });
-        if (occ / 2 < 1 || occ == 1) break;
-    }
-    // two rounds over the candidates, each candidate's best round counts
-    // (graph replays: the launch count of each schedule is part of its cost)
     for (mp_tuning& c : cand) c.frame_ms = 1e30;
     for (int round = 0; round < 2; ++round) {
         for (mp_tuning& c : cand) {
@@ -1337,10 +1331,47 @@ extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
             c.frame_ms = std::min(c.frame_ms, ms);
         }
     }
-    mp_tuning best{ -1, 0, 0, 1e30 };
-    for (const mp_tuning& c : cand)
-        if (c.frame_ms < best.frame_ms) best = c;
-    if (best.split_engine < 0) return fail(ctx, MP_E_STATE, "no tuning candidate ran");
+    int best = -1;
+    for (int i = 0; i < (int)cand.size(); ++i)
+        if (cand[i].frame_ms < 1e29 && (best < 0 || cand[i].frame_ms < cand[best].frame_ms)) best = i;
+    return best;
+}
+
+extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
+{
+    if (!ctx || replays < 1) return MP_E_INVALID;
+    if (!ctx->frame_valid || !ctx->device_driven)
+        return fail(ctx, MP_E_STATE, "autotune needs a finished LOCAL or BALANCED frame or batch");
+    mp_tuning cur;
+    mp_get_tuning(ctx, &cur);
+    // stage 1: raster schedule (graph replays include the launch costs)
+    const int occ = raster_grid(ctx->cfg.reyes_patch_size, ctx->num_cus, ctx->prm) / ctx->num_cus;
+    std::vector<mp_tuning> cand;
+    for (int per_cu : { occ, std::max(1, occ / 2) }) {
+        for (int chunk : { 0, -1 }) {
+            mp_tuning c = cur;
+            c.raster_per_cu = per_cu;
+            c.raster_chunk = chunk;
+            cand.push_back(c);
+        }
+        if (occ < 2) break;
+    }
+    int b = pick_fastest(ctx, cand, replays);
+    if (b < 0) return fail(ctx, MP_E_STATE, "no tuning candidate ran");
+    cur = cand[b];
+    // stage 2: split schedule
+    cand.clear();
+    for (int e : { 0, 1 })
+        for (int ws : { 0, 1 }) {
+            if (e == 1 && ws == 1) continue;
+            mp_tuning c = cur;
+            c.split_engine = e;
+            c.work_sharing = ws;
+            cand.push_back(c);
+        }
+    b = pick_fastest(ctx, cand, replays);
+    if (b < 0) return fail(ctx, MP_E_STATE, "no tuning candidate ran");
+    const mp_tuning best = cand[b];
     int rc;
     if ((rc = apply_tuning(ctx, best))) return rc;
     if (chosen) *chosen = best;

@@ -714,13 +714,25 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     uint32_t slow = 0, natom = 0, uncached = 0;
     __shared__ uint32_t s_next;
 
-    // chunked dynamic scheduling: one atomic per `chunk` ranges keeps the
-    // shared counter off the critical path for million-range batches
-    const uint32_t chunk = max(1u, min(64u, total / (gridDim.x * 16u)));
-    if (tid == 0) s_next = atomicAdd(&fc->raster_next, chunk);
-    __syncthreads();
-    uint32_t r = s_next;
-    uint32_t r_end = min(r + chunk, total);
+    // Range schedule.  Static (prm.raster_chunk == 0): work-group g takes
+    // ranges g, g + grid, ... with no atomics, the next range known ahead.
+    // Dynamic: chunks claimed from one counter, one atomic per chunk (a
+    // returning atomic on one address serialises across the device, so the
+    // chunk must stay large against the range count); raster_chunk < 0
+    // sizes it from the range count, > 0 fixes it.
+    const bool dyn = prm.raster_chunk != 0;
+    const uint32_t chunk = prm.raster_chunk > 0 ? (uint32_t)prm.raster_chunk
+                                                : max(8u, min(64u, total / (gridDim.x * 4u)));
+    uint32_t r, r_end;
+    if (dyn) {
+        if (tid == 0) s_next = atomicAdd(&fc->raster_next, chunk);
+        __syncthreads();
+        r = s_next;
+        r_end = min(r + chunk, total);
+    } else {
+        r = blockIdx.x;
+        r_end = total;
+    }
     if (r < total) {
         const RangeRec rec0 = canon[r];
         const DevObject& ob0 = objs[rec0.obj_depth & 0xffffffu];
@@ -733,8 +745,8 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
         // next range (the following chunk is claimed one range ahead) and its
         // control points: loaded now, stored to the other LDS buffer after
         // this range, so the load latency hides behind this range's work
-        uint32_t rn = r + 1, rn_end = r_end;
-        if (rn >= r_end) {
+        uint32_t rn = dyn ? r + 1 : r + gridDim.x, rn_end = r_end;
+        if (dyn && rn >= r_end) {
             if (tid == 0) s_next = atomicAdd(&fc->raster_next, chunk);
             __syncthreads();
             rn = s_next;

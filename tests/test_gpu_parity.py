@@ -81,17 +81,17 @@ def _frame_of(r):
 
 
 def test_every_scheduling_gives_the_same_frame(oracle):
-    """Split engine and work sharing change only the schedule (BALANCED)."""
+    """Split engine, work sharing and the raster range schedule change only
+    the schedule (BALANCED)."""
     w = scenes.c4_gregory(planes=8, quads=128, width=960, height=540)
     print("case c4_gregory small BALANCED tunings", flush=True)
     ref = oracle.render(w)[0]
     r, got = render_plain(w)
     assert_frame_equal(got, ref, "c4-small/default")
-    for engine in (0, 1):
-        for sharing in (0, 1):
-            r.set_tuning(split_engine=engine, work_sharing=sharing)
-            r.replay(2, use_graph=True)
-            assert_frame_equal(_frame_of(r), ref, "c4-small/engine%d-sharing%d" % (engine, sharing))
+    for engine, sharing, chunk in ((0, 0, 0), (0, 1, -1), (1, 0, 0), (1, 1, -1), (0, 0, 7)):
+        r.set_tuning(split_engine=engine, work_sharing=sharing, raster_chunk=chunk)
+        r.replay(2, use_graph=True)
+        assert_frame_equal(_frame_of(r), ref, "c4-small/engine%d-sharing%d-chunk%d" % (engine, sharing, chunk))
     r.close()
 
 
