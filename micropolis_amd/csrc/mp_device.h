@@ -186,7 +186,13 @@ MP_DEV float lattice(int32_t x, int32_t y, int32_t z, uint32_t seed)
 MP_DEV float lattice_wave(int32_t x, int32_t y, int32_t z, uint32_t seed)
 {
     const uint32_t h = (uint32_t)__builtin_amdgcn_readfirstlane((int)noise_hash(x, y, z, seed));
-    return (float)(h >> 8) * (1.0f / 8388608.0f) - 1.0f;
+    // m * 2^-23 - 1 for m = h >> 8 < 2^24, built on the scalar unit: the
+    // float with mantissa bits (m mod 2^23) and exponent 0 is 1 + (m mod 2^23)
+    // * 2^-23, so subtracting 2 (m < 2^23) or 1 (m >= 2^23) gives the value
+    // exactly, with one vector instruction instead of three
+    const uint32_t m = h >> 8;
+    const float one_plus = __uint_as_float(0x3f800000u | (m & 0x7fffffu));
+    return one_plus - ((m >> 23) ? 1.0f : 2.0f);
 }
 MP_DEV float fade(float t) { return (t * t) * (3.0f - 2.0f * t); }
 MP_DEV float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
@@ -232,26 +238,41 @@ MP_DEV float noise3(const DevParams& prm, float x, float y, float z)
     return acc / norm;
 }
 
-// Unit displacement direction at (u,v): unit(cross(dP/dv, dP/du)), zero
-// where the cross product vanishes (Bezier patches).
-MP_DEV void bezier_unit_normal(const float4* __restrict__ cp, float u, float v, float n[3])
+// Column sums of the displacement normal at u, i ascending:
+// col[j] = sum_i a_i(u) P_ij, col[4 + j] = sum_i a_i'(u) P_ij (xyz; w = 0).
+// Every grid point of one u column shares them.
+MP_DEV float4 normal_column(const float4* __restrict__ cp, float u, int k)
 {
-    float su = 1.0f - u, sv = 1.0f - v;
-    float a[4], b[4], da[4], db[4];
-    bern_u(u, su, a);
+    const float su = 1.0f - u;
+    float w[4];
+    if (k < 4) bern_u(u, su, w);
+    else       dbern(u, su, w);
+    const int j = k & 3;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float4 P = cp[4 * i + j];
+        x = x + w[i] * P.x; y = y + w[i] * P.y; z = z + w[i] * P.z;
+    }
+    return make_float4(x, y, z, 0.0f);
+}
+
+// Unit displacement direction from the column sums at v:
+// dP/du = sum_j b_j(v) col[4 + j], dP/dv = sum_j b_j'(v) col[j] (j
+// ascending), n = unit(cross(dP/dv, dP/du)), zero where it vanishes.
+MP_DEV void normal_from_columns(const float4* __restrict__ col, float v, float n[3])
+{
+    const float sv = 1.0f - v;
+    float b[4], db[4];
     bern_v(v, sv, b);
-    dbern(u, su, da);
     dbern(v, sv, db);
     float pux = 0.f, puy = 0.f, puz = 0.f, pvx = 0.f, pvy = 0.f, pvz = 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float4 P = cp[4 * i + j];
-            float wu = da[i] * b[j], wv = a[i] * db[j];
-            pux = pux + wu * P.x; puy = puy + wu * P.y; puz = puz + wu * P.z;
-            pvx = pvx + wv * P.x; pvy = pvy + wv * P.y; pvz = pvz + wv * P.z;
-        }
+    for (int j = 0; j < 4; ++j) {
+        const float4 S = col[j], D = col[4 + j];
+        pux = pux + b[j] * D.x; puy = puy + b[j] * D.y; puz = puz + b[j] * D.z;
+        pvx = pvx + db[j] * S.x; pvy = pvy + db[j] * S.y; pvz = pvz + db[j] * S.z;
+    }
     float nx = pvy * puz - pvz * puy;
     float ny = pvz * pux - pvx * puz;
     float nz_ = pvx * puy - pvy * pux;
@@ -261,6 +282,15 @@ MP_DEV void bezier_unit_normal(const float4* __restrict__ cp, float u, float v, 
         float inv = 1.0f / sqrtf(len2);
         n[0] = nx * inv; n[1] = ny * inv; n[2] = nz_ * inv;
     }
+}
+
+// Unit displacement direction at (u,v) (Bezier patches).
+MP_DEV void bezier_unit_normal(const float4* __restrict__ cp, float u, float v, float n[3])
+{
+    float4 col[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) col[k] = normal_column(cp, u, k);
+    normal_from_columns(col, v, n);
 }
 
 // Lattice values of one noise octave cached for a box of lattice cells.
@@ -304,36 +334,26 @@ MP_DEV void displaced_point(const float4* __restrict__ cp, int type, float u, fl
 {
     eval_patch(cp, type, u, v, out);
     if (!prm.displacement || type != MP_TYPE_BEZIER) return;
-    float su = 1.0f - u, sv = 1.0f - v;
-    float a[4], b[4], da[4], db[4];
-    bern_u(u, su, a);
-    bern_v(v, sv, b);
-    dbern(u, su, da);
-    dbern(v, sv, db);
-    float pux = 0.f, puy = 0.f, puz = 0.f, pvx = 0.f, pvy = 0.f, pvz = 0.f;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float4 P = cp[4 * i + j];
-            float wu = da[i] * b[j], wv = a[i] * db[j];
-            pux = pux + wu * P.x; puy = puy + wu * P.y; puz = puz + wu * P.z;
-            pvx = pvx + wv * P.x; pvy = pvy + wv * P.y; pvz = pvz + wv * P.z;
-        }
-    // n = cross(pv, pu)
-    float nx = pvy * puz - pvz * puy;
-    float ny = pvz * pux - pvx * puz;
-    float nz_ = pvx * puy - pvy * pux;
-    float len2 = dot3(nx, ny, nz_, nx, ny, nz_);
-    float hx = 0.f, hy = 0.f, hz = 0.f;
-    if (len2 > 0.0f) {
-        float inv = 1.0f / sqrtf(len2);
-        hx = nx * inv; hy = ny * inv; hz = nz_ * inv;
-    }
+    float h[3];
+    bezier_unit_normal(cp, u, v, h);
     float d = amp * noise3(prm, out[0], out[1], out[2]);
-    out[0] = out[0] + d * hx;
-    out[1] = out[1] + d * hy;
-    out[2] = out[2] + d * hz;
+    out[0] = out[0] + d * h[0];
+    out[1] = out[1] + d * h[1];
+    out[2] = out[2] + d * h[2];
+}
+
+// Same with the normal's column sums precomputed (col: 8 float4 of the
+// point's u column).
+MP_DEV void displaced_point_cols(const float4* __restrict__ cp, const float4* __restrict__ col,
+                                 float u, float v, const DevParams& prm, float amp, float out[4])
+{
+    eval_bezier(cp, u, v, out);
+    float h[3];
+    normal_from_columns(col, v, h);
+    float d = amp * noise3(prm, out[0], out[1], out[2]);
+    out[0] = out[0] + d * h[0];
+    out[1] = out[1] + d * h[1];
+    out[2] = out[2] + d * h[2];
 }
 
 // ---- bound() (kernels/bound_n_split.h:43-100) with RES = 3 ----

@@ -703,6 +703,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     __shared__ float s_box[Q / 64][6];
     __shared__ int s_geo[kCachedOctaves][6];
     __shared__ float s_lat[kCachedOctaves][kCacheEntries];
+    __shared__ float4 s_col[MODE == 1 ? G1 * 8 : 1];   // normal column sums per u column
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -741,6 +742,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     }
     __syncthreads();
     int buf = 0;
+    float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     while (r < total) {
         // next range (the following chunk is claimed one range ahead) and its
         // control points: loaded now, stored to the other LDS buffer after
@@ -752,7 +754,6 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
             rn = s_next;
             rn_end = min(rn + chunk, total);
         }
-        float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         bool have_pf = false;
         if (rn < total) {
             const RangeRec nrec = canon[rn];
@@ -769,6 +770,16 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
         const DevView& vw = views[ob.view];
 
         // ---- dice (dice.cl:38-79) ----
+        const bool disp_cols = MODE == 1 && ob.type == MP_TYPE_BEZIER;
+        if (disp_cols) {
+            // displacement normal: column sums once per u column
+            for (int t = tid; t < G1 * 8; t += Q) {
+                const int c = t >> 3;
+                const float uc = rec.umin + (rec.umax - rec.umin) * ((float)c / (float)N);
+                s_col[t] = normal_column(s_cp, uc, t & 7);
+            }
+            __syncthreads();
+        }
         if (MODE != 2 || ob.type != MP_TYPE_BEZIER) {
             for (int g = tid; g < G; g += Q) {
                 const int nu = g % G1, nv = g / G1;
@@ -776,7 +787,8 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
                 const float uu = rec.umin + (rec.umax - rec.umin) * tu;
                 const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
                 float P[4], e[4], q[4];
-                if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
+                if (disp_cols) displaced_point_cols(s_cp, s_col + 8 * nu, uu, vv, prm, ob.disp_amplitude, P);
+                else if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
                 else eval_patch(s_cp, ob.type, uu, vv, P);
                 mat4_mul(ob.mv, P, e);
                 mat4_mul(vw.proj, e, q);
@@ -884,6 +896,12 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
         }
         __syncthreads();
 
+        // The next range's control points go to the other buffer now: the
+        // load was issued before the dice and has landed, and no atomic of
+        // this range is in flight yet (the wait counter for vector memory
+        // is in order, so a later wait would also wait for the atomics).
+        if (have_pf) s_cpb[buf ^ 1][tid] = pf;
+
         // ---- shade (reyes.cl:84-199); one wave = one 8x8 block ----
         const float4 p0 = s_pos[g0], p1 = s_pos[g0 + 1], p2 = s_pos[g0 + G1], p3 = s_pos[g0 + G1 + 1];
         const int2 x0 = s_pxl[g0], x1 = s_pxl[g0 + 1], x2 = s_pxl[g0 + G1], x3 = s_pxl[g0 + G1 + 1];
@@ -975,7 +993,6 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
                 }
             }
         }
-        if (have_pf) s_cpb[buf ^ 1][tid] = pf;
         buf ^= 1;
         r = rn;
         r_end = rn_end;

@@ -279,21 +279,32 @@ void or_displaced_point(const float* cps, int type, int pid, float u, float v,
     or_eval_patch(cps, type, pid, u, v, out);
     if (!prm->displacement || type != OR_BEZIER) return;
     const float* cp = cps + (long)pid * 16 * 4;
+    /* tangents of the displacement normal, summed over i first (column sums
+     * S_j = sum_i a_i(u) P_ij, D_j = sum_i a_i'(u) P_ij, i ascending), then
+     * dP/du = sum_j b_j(v) D_j and dP/dv = sum_j b_j'(v) S_j, j ascending */
     float su = 1.0f - u, sv = 1.0f - v;
     float a[4], b[4], da[4], db[4];
     bern_u(u, su, a);
     bern_v(v, sv, b);
     dbern(u, su, da);
     dbern(v, sv, db);
-    float pu[3] = { 0, 0, 0 }, pv[3] = { 0, 0, 0 };
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) {
-            const float* P = cp + 4 * (4 * i + j);
-            float wu = da[i] * b[j], wv = a[i] * db[j];
-            for (int k = 0; k < 3; ++k) {
-                pu[k] = pu[k] + wu * P[k];
-                pv[k] = pv[k] + wv * P[k];
+    float S[4][3], D[4][3];
+    for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 3; ++k) {
+            float sacc = 0.0f, dacc = 0.0f;
+            for (int i = 0; i < 4; ++i) {
+                const float P = cp[4 * (4 * i + j) + k];
+                sacc = sacc + a[i] * P;
+                dacc = dacc + da[i] * P;
             }
+            S[j][k] = sacc;
+            D[j][k] = dacc;
+        }
+    float pu[3] = { 0, 0, 0 }, pv[3] = { 0, 0, 0 };
+    for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 3; ++k) {
+            pu[k] = pu[k] + b[j] * D[j][k];
+            pv[k] = pv[k] + db[j] * S[j][k];
         }
     float n[3];
     cross3(pv, pu, n);

@@ -105,3 +105,50 @@ def test_value_noise_lattice_and_bounds(oracle):
     rng = np.random.default_rng(5)
     v4 = np.array([lib.or_noise(C.byref(prm4), *map(float, p)) for p in rng.uniform(-2, 2, (200, 3))])
     assert np.all(np.abs(v4) <= 1.0)
+
+
+def _displaced(oracle, cps, u, v, prm):
+    out = np.zeros(4, np.float32)
+    oracle.load().or_displaced_point(oracle.fptr(cps), 0, 0, u, v, C.byref(prm), oracle.fptr(out))
+    return out
+
+
+def _unit_normal64(P, u, v):
+    """FP64 unit(cross(dP/dv, dP/du)) of the Bernstein patch (i along u)."""
+    b = lambda t: np.array([(1 - t) ** 3, 3 * t * (1 - t) ** 2, 3 * t * t * (1 - t), t ** 3])
+    db = lambda t: np.array([-3 * (1 - t) ** 2, 3 * (1 - t) * (1 - 3 * t), 3 * t * (2 - 3 * t), 3 * t * t])
+    G = P.reshape(4, 4, 3).astype(np.float64)
+    pu = np.einsum("i,j,ijc->c", db(u), b(v), G)
+    pv = np.einsum("i,j,ijc->c", b(u), db(v), G)
+    n = np.cross(pv, pu)
+    return n / np.linalg.norm(n)
+
+
+def test_displacement_follows_the_surface_normal(oracle):
+    """P + amp * noise(P) * n with n the unit normal cross(dP/dv, dP/du)."""
+    lib = oracle.load()
+    prm = oracle.make_params(displacement=1, displacement_amplitude=0.05, displacement_frequency=3.0,
+                             displacement_octaves=4, displacement_seed=11)
+    # planar patch z = 0.3: displacement along z only, x and y unchanged
+    P = np.zeros((1, 16, 3), np.float32)
+    for i in range(4):
+        for j in range(4):
+            P[0, 4 * i + j] = (i / 3.0, j / 3.0, 0.3)
+    cps = oracle.as_cps(P)
+    for u, v in ((0.2, 0.7), (0.5, 0.5), (0.9, 0.1)):
+        base = _eval(oracle, cps, 0, 0, u, v)
+        d = _displaced(oracle, cps, u, v, prm)
+        assert d[0] == base[0] and d[1] == base[1]
+        nz = lib.or_noise(C.byref(prm), float(base[0]), float(base[1]), float(base[2]))
+        # cross(dP/dv, dP/du) of this parametrisation points along -z
+        np.testing.assert_allclose(d[2] - base[2], -0.05 * nz, rtol=1e-5, atol=1e-7)
+    # curved patches: direction and length against the FP64 normal
+    rng = np.random.default_rng(12)
+    for trial in range(20):
+        P = rng.uniform(-1, 1, (1, 16, 3)).astype(np.float32)
+        cps = oracle.as_cps(P)
+        u, v = rng.uniform(0.05, 0.95, 2).astype(np.float32)
+        base = _eval(oracle, cps, 0, 0, u, v)
+        d = (_displaced(oracle, cps, u, v, prm) - base)[:3].astype(np.float64)
+        nz = lib.or_noise(C.byref(prm), float(base[0]), float(base[1]), float(base[2]))
+        np.testing.assert_allclose(d, 0.05 * nz * _unit_normal64(P[0], float(u), float(v)), rtol=0, atol=2e-6)
