@@ -220,6 +220,8 @@ typedef struct mp_tuning {
     int32_t raster_chunk;    /* raster range schedule: 0 static stride (no
                                 atomics), -1 dynamic chunks sized from the
                                 range count, > 0 dynamic chunks of that size */
+    int32_t split_levels;    /* split_engine 1: level launches before the
+                                persistent tail (0 = as many as the frame needs) */
     double  frame_ms;        /* mp_autotune: replay time of the chosen setting */
 } mp_tuning;
 int mp_get_tuning(mp_context* ctx, mp_tuning* tuning);
@@ -228,7 +230,8 @@ int mp_set_tuning(mp_context* ctx, const mp_tuning* tuning);
 /* Autotuner: replay the last finished LOCAL/BALANCED frame or batch under
  * candidate settings, `replays` graph replays each (best of two rounds):
  * first the raster schedule (work-groups per CU x static / dynamic ranges),
- * then the split (engine x work sharing); keep the fastest and report it. */
+ * then the split (persistent with / without work sharing, or a number of
+ * level launches before the persistent tail); keep the fastest. */
 int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen);
 
 /* Replay the last LOCAL/BALANCED frame n times with HIP events between its

@@ -133,8 +133,9 @@ struct mp_context {
     uint32_t* d_level_cnt = nullptr; uint32_t* h_level_cnt = nullptr;
     int level_launches = 0;        // levels launched before the persistent tail
     int level_grid_size = 0;
-    int level_lanes = 9;           // lanes per range in the level kernels (9 or 1)
+    int level_lanes = 1;           // lanes per range in the level kernels (1 or 9; 1 measured faster)
     int level_per_cu = 1;          // level-kernel work-groups per CU (0: occupancy)
+    int level_limit = 0;           // level launches before the persistent tail (0: all needed)
     int spill_at = 192;
     int linger = -1;             // groups that wait for spilled work (-1: one per CU)
     // CPU split method
@@ -354,6 +355,7 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     if (const char* e = getenv("MP_CPU_THREADS")) ctx->cpu_threads = atoi(e);
     if (const char* e = getenv("MP_LEVEL_LANES")) ctx->level_lanes = atoi(e) == 1 ? 1 : 9;
     if (const char* e = getenv("MP_LEVEL_PER_CU")) ctx->level_per_cu = atoi(e);
+    if (const char* e = getenv("MP_SPLIT_LEVELS")) ctx->level_limit = std::max(0, atoi(e));
     ctx->level_launches = cfg->max_split_depth;
     if (const char* e = getenv("MP_RASTER_PER_CU")) ctx->raster_grid_size = ctx->num_cus * std::max(1, atoi(e));
     memset(&ctx->stats, 0, sizeof(ctx->stats));
@@ -948,6 +950,7 @@ static int update_level_launches(mp_context* ctx, int64_t* max_patches)
         if (max_patches) *max_patches = std::max<int64_t>(*max_patches, ctx->h_level_cnt[L]);
     }
     ctx->level_launches = std::min(ctx->cfg.max_split_depth, deepest + 2);
+    if (ctx->level_limit > 0) ctx->level_launches = std::min(ctx->level_launches, ctx->level_limit);
     return MP_OK;
 }
 
@@ -1281,6 +1284,7 @@ extern "C" int mp_get_tuning(mp_context* ctx, mp_tuning* t)
     t->work_sharing = ctx->work_sharing ? 1 : 0;
     t->raster_per_cu = ctx->raster_grid_size / std::max(1, ctx->num_cus);
     t->raster_chunk = ctx->prm.raster_chunk;
+    t->split_levels = ctx->level_limit;
     t->frame_ms = 0.0;
     return MP_OK;
 }
@@ -1293,8 +1297,11 @@ static int apply_tuning(mp_context* ctx, const mp_tuning& t)
     ctx->work_sharing = t.work_sharing != 0;
     ctx->raster_grid_size = ctx->num_cus * std::max(1, t.raster_per_cu);
     ctx->prm.raster_chunk = t.raster_chunk;
+    ctx->level_limit = std::max(0, t.split_levels);
     ctx->level_grid_size = 0;
-    if (ctx->split_engine == 1) ctx->level_launches = ctx->cfg.max_split_depth;
+    if (ctx->split_engine == 1)
+        ctx->level_launches = ctx->level_limit > 0 ? std::min(ctx->level_limit, ctx->cfg.max_split_depth)
+                                                   : ctx->cfg.max_split_depth;
     invalidate_graph(ctx);
     if (!ctx->frame_valid || !ctx->device_driven) return MP_OK;
     int rc;
@@ -1311,7 +1318,7 @@ extern "C" int mp_set_tuning(mp_context* ctx, const mp_tuning* t)
 {
     if (!ctx || !t) return MP_E_INVALID;
     if (t->split_engine < 0 || t->split_engine > 1 || t->raster_per_cu < 0 || t->raster_per_cu > 64 ||
-        t->raster_chunk < -1 || t->raster_chunk > 4096)
+        t->raster_chunk < -1 || t->raster_chunk > 4096 || t->split_levels < 0 || t->split_levels > 23)
         return fail(ctx, MP_E_INVALID, "tuning out of range");
     mp_tuning u = *t;
     if (u.raster_per_cu == 0) u.raster_per_cu = raster_grid(ctx->cfg.reyes_patch_size, ctx->num_cus, ctx->prm) / ctx->num_cus;
@@ -1361,14 +1368,20 @@ extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
     cur = cand[b];
     // stage 2: split schedule
     cand.clear();
-    for (int e : { 0, 1 })
-        for (int ws : { 0, 1 }) {
-            if (e == 1 && ws == 1) continue;
-            mp_tuning c = cur;
-            c.split_engine = e;
-            c.work_sharing = ws;
-            cand.push_back(c);
-        }
+    for (int ws : { 0, 1 }) {           // persistent split
+        mp_tuning c = cur;
+        c.split_engine = 0;
+        c.work_sharing = ws;
+        c.split_levels = 0;
+        cand.push_back(c);
+    }
+    for (int lv : { 0, 3, 5 }) {         // level launches, then the persistent tail
+        mp_tuning c = cur;
+        c.split_engine = 1;
+        c.work_sharing = 0;
+        c.split_levels = lv;
+        cand.push_back(c);
+    }
     b = pick_fastest(ctx, cand, replays);
     if (b < 0) return fail(ctx, MP_E_STATE, "no tuning candidate ran");
     const mp_tuning best = cand[b];
