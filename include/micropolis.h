@@ -180,6 +180,19 @@ int mp_cpu_bound_n_split(const mp_config* cfg, const float* xyz, size_t n_points
                          int64_t capacity, int64_t* count, int32_t* pass_counts,
                          int32_t max_passes, int32_t* n_passes, int64_t* n_bounds);
 
+/* Renderer::dump_trace (src/Reyes/Renderer.h:44): write the stages of the
+ * last frame (run once more with events between its stages for LOCAL and
+ * BALANCED frames; the whole frame otherwise) in the reference's trace
+ * format, one line per stage,
+ *   name@queue:queued:submit:start:end:id:dependencies
+ * with host-clock nanoseconds (src/CL/Device.cpp:275-289). */
+int mp_dump_trace(mp_context* ctx, const char* path);
+
+/* Statistics::dump_stats (src/base/Statistics.cpp:196-218): `key = value;`
+ * lines for the last frame; opencl_mem is the device memory this context
+ * holds (by use), opengl_mem 0, bound_n_split_balance empty. */
+int mp_dump_stats(mp_context* ctx, const char* path);
+
 /* Statistics (src/base/Statistics.cpp:160-218) of the last frame. */
 int mp_get_stats(mp_context* ctx, mp_stats* stats);
 

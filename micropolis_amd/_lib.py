@@ -101,6 +101,8 @@ EXPORTS = [
     ("mp_backward", C.c_int, [_P, _P, _P, _P, _P]),
     ("mp_batch_read_device", C.c_int, [_P, C.c_int32, _P, _P]),
     ("mp_update_patches_device", C.c_int, [_P, C.c_uint64, _P, C.c_size_t]),
+    ("mp_dump_trace", C.c_int, [_P, C.c_char_p]),
+    ("mp_dump_stats", C.c_int, [_P, C.c_char_p]),
     ("mp_get_tuning", C.c_int, [_P, C.POINTER(mp_tuning)]),
     ("mp_set_tuning", C.c_int, [_P, C.POINTER(mp_tuning)]),
     ("mp_autotune", C.c_int, [_P, C.c_int32, C.POINTER(mp_tuning)]),

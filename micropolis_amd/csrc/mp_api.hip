@@ -18,6 +18,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <fstream>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -1221,6 +1223,96 @@ extern "C" int mp_get_stats(mp_context* ctx, mp_stats* stats)
     if (!ctx || !stats) return MP_E_INVALID;
     *stats = ctx->stats;
     return MP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// trace and statistics files in the reference's formats
+
+// Device memory held by the context, by use (Statistics::opencl_memory_by_use).
+static std::vector<std::pair<std::string, size_t>> memory_by_use(const mp_context* ctx)
+{
+    size_t patches = 0, fb = 0;
+    for (const auto& kv : ctx->patches)
+        patches += (size_t)kv.second.n_patches * (kv.second.type == MP_GREGORY ? 20 : 16) * sizeof(float4);
+    for (const ViewBuffers& vb : ctx->views) {
+        const size_t px = (size_t)vb.width * vb.height;
+        fb += (size_t)vb.grid_w * vb.grid_h * 64 * 8 + px * (4 + 4 + 4) + (vb.rgba32 ? px * 16 : 0);
+    }
+    const size_t split = ctx->queue_entries * sizeof(RangeRec) + (size_t)ctx->queue_groups_cap * 4 +
+                         ctx->shared_entries * sizeof(RangeRec) + (size_t)ctx->chunk_cap * (64 * sizeof(RangeRec) + 8) +
+                         (size_t)ctx->level_cap * 2 * sizeof(RangeRec) + (ctx->seg_a_cap + ctx->seg_b_cap) * sizeof(RangeRec) +
+                         ctx->flags_cap + ctx->blk_cap * 4;
+    const size_t lists = (size_t)ctx->list_cap * (3 * sizeof(RangeRec) + 8) + ctx->quad_cap * sizeof(float) +
+                         (size_t)ctx->gpid_cap * 12 + ctx->visible_cap * 4;
+    return { { "patch data", patches }, { "framebuffer", fb }, { "bound&split", split }, { "range lists", lists } };
+}
+
+extern "C" int mp_dump_stats(mp_context* ctx, const char* path)
+{
+    if (!ctx || !path) return MP_E_INVALID;
+    if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame");
+    std::ofstream fs(path);
+    if (!fs) return fail(ctx, MP_E_INVALID, std::string("cannot write ") + path);
+    const auto mem = memory_by_use(ctx);
+    size_t total = 0;
+    for (const auto& m : mem) total += m.second;
+    int64_t inputs = 0;
+    for (const ObjectReq& o : ctx->objects) inputs += ctx->patches[o.handle].n_patches;
+    const mp_stats& st = ctx->stats;
+    fs << "opencl_mem = " << total << ";\n";
+    for (const auto& m : mem) fs << "opencl_mem@" << m.first << " = " << m.second << ";\n";
+    fs << "opengl_mem = 0;\n";
+    fs << "max_patches = " << st.max_patches << ";\n";
+    fs << "patches_per_frame = " << st.patches_per_frame << ";\n";
+    fs << "processed_patches_per_frame = " << st.bounded_patches << ";\n";
+    fs << "pass_count = " << st.pass_count << ";\n";
+    fs << "total_input_patches = " << inputs << ";\n";
+    fs << "bound_n_split_balance = ;";
+    return fs.good() ? MP_OK : fail(ctx, MP_E_INVALID, "write failed");
+}
+
+extern "C" int mp_dump_trace(mp_context* ctx, const char* path)
+{
+    if (!ctx || !path) return MP_E_INVALID;
+    if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame");
+    std::ofstream fs(path);
+    if (!fs) return fail(ctx, MP_E_INVALID, std::string("cannot write ") + path);
+    const uint64_t base = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                              std::chrono::steady_clock::now().time_since_epoch()).count();
+    struct Rec { const char* name; double t0, t1; };
+    std::vector<Rec> recs;
+    if (ctx->device_driven) {
+        // one more run of the frame with an event between its stages
+        hipEvent_t ev[5] = { nullptr, nullptr, nullptr, nullptr, nullptr };
+        int rc = MP_OK;
+        for (auto& e : ev)
+            if (hipEventCreate(&e) != hipSuccess) rc = fail(ctx, MP_E_HIP, "hipEventCreate failed");
+        if (rc == MP_OK) rc = launch_device_frame(ctx, ev);
+        if (rc == MP_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, MP_E_HIP, "sync failed");
+        // names from the reference's trace categories (tools/trace_summary.py):
+        // split -> subdivision, canonical order (scan + scatter) -> subdivision,
+        // fused dice/shade/sample -> sample, resolve (also clears) -> clear
+        static const char* names[4] = { "bound & split", "accumulate", "sample", "clear framebuffer" };
+        for (int k = 0; k < 4 && rc == MP_OK; ++k) {
+            float a = 0.0f, b = 0.0f;
+            hipEventElapsedTime(&a, ev[0], ev[k]);
+            hipEventElapsedTime(&b, ev[0], ev[k + 1]);
+            recs.push_back(Rec{ names[k], a, b });
+        }
+        for (auto& e : ev)
+            if (e) hipEventDestroy(e);
+        if (rc) return rc;
+    } else {
+        recs.push_back(Rec{ "frame", 0.0, ctx->stats.frame_ms });
+    }
+    // name@queue:queued:submit:start:end:id:dependencies (src/CL/Device.cpp:275-289), ns
+    for (size_t k = 0; k < recs.size(); ++k) {
+        const uint64_t t0 = base + (uint64_t)(recs[k].t0 * 1e6), t1 = base + (uint64_t)(recs[k].t1 * 1e6);
+        fs << recs[k].name << "@device:" << t0 << ":" << t0 << ":" << t0 << ":" << t1 << ":" << k << ":";
+        if (k > 0) fs << (k - 1);
+        fs << "\n";
+    }
+    return fs.good() ? MP_OK : fail(ctx, MP_E_INVALID, "write failed");
 }
 
 extern "C" int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* ms_per_frame)

@@ -226,6 +226,15 @@ class Renderer:
         self._check(self._lib.mp_replay_stages(self._ctx, int(n), ms))
         return dict(zip(("split", "order", "raster", "resolve"), list(ms)))
 
+    def dump_trace(self, path):
+        """Renderer::dump_trace: stages of the last frame in the reference's
+        trace format."""
+        self._check(self._lib.mp_dump_trace(self._ctx, str(path).encode()))
+
+    def dump_stats(self, path):
+        """Statistics::dump_stats format for the last frame."""
+        self._check(self._lib.mp_dump_stats(self._ctx, str(path).encode()))
+
     def tuning(self):
         t = _lib.mp_tuning()
         self._check(self._lib.mp_get_tuning(self._ctx, C.byref(t)))
