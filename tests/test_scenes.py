@@ -48,3 +48,35 @@ def test_gregory_planes_have_distinct_face_points():
     # f- (16..19) is perturbed away from f+ (12..15), so the rational
     # interior terms of the Gregory evaluation matter
     assert np.abs(g[:, 16:20] - g[:, 12:16]).max() > 1e-3
+
+
+def test_batch_split_depths_span_0_to_12(oracle):
+    w = scenes.c3_batch()
+    hist = np.zeros(24, int)
+    prm = oracle.make_params(w.options)
+    for v in w.views:
+        o = v.objects[0]
+        view = oracle.make_view(v.projection.fovy, v.projection.near, *v.projection.viewport)
+        r, _, _, _ = oracle.split("LOCAL", oracle.as_cps(o.patches), 0, o.matrix, view, prm,
+                                  float(w.options["bound_n_split_limit"]))
+        hist += np.bincount(r["depth"], minlength=24)[:24]
+    assert (hist[:13] > 0).all() and hist[13:].sum() == 0
+
+
+def test_gregory_planes_reach_depth_complexity_32():
+    """Layers per pixel from the projected plane extents (each plane is a
+    grid of quads tiling its rectangle)."""
+    w = scenes.c4_gregory()
+    o = w.views[0].objects[0]
+    W, H = 1920, 1080
+    f = (H / 2) / np.tan(np.radians(30))
+    pts = o.patches[:, :4, :].reshape(-1, 3) @ o.matrix[:3, :3].T + o.matrix[:3, 3]
+    pts = pts.reshape(32, -1, 3)
+    cnt = np.zeros((H, W), np.int32)
+    for k in range(32):
+        sx = pts[k, :, 0] / -pts[k, :, 2] * f + W / 2
+        sy = pts[k, :, 1] / -pts[k, :, 2] * f + H / 2
+        x0, x1 = int(max(0, sx.min())), int(min(W, sx.max()))
+        y0, y1 = int(max(0, sy.min())), int(min(H, sy.max()))
+        cnt[y0:y1, x0:x1] += 1
+    assert cnt.max() == 32 and (cnt >= 24).mean() > 0.05
