@@ -336,7 +336,7 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     // hashing the lattice per use measured faster than the per-range LDS cache
     ctx->prm.noise_cache = 0;
     if (const char* e = getenv("MP_NOISE_CACHE")) ctx->prm.noise_cache = atoi(e);
-    ctx->prm.noise_wave_cells = 1;
+    ctx->prm.noise_wave_cells = 2;   // lane window (0 per lane, 1 wave-uniform cell)
     ctx->prm.raster_chunk = 0;   // static range stride (see mp_tuning)
     if (const char* e = getenv("MP_RASTER_CHUNK")) ctx->prm.raster_chunk = atoi(e);
     if (const char* e = getenv("MP_NOISE_WAVE_CELLS")) ctx->prm.noise_wave_cells = atoi(e);

@@ -197,7 +197,43 @@ MP_DEV float lattice_wave(int32_t x, int32_t y, int32_t z, uint32_t seed)
 MP_DEV float fade(float t) { return (t * t) * (3.0f - 2.0f * t); }
 MP_DEV float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
 
-MP_DEV float vnoise(float x, float y, float z, uint32_t seed, bool wave_cells = false)
+// Value of v held by lane byte_addr / 4 of the wave (ds_bpermute_b32).
+MP_DEV float lane_value(int byte_addr, float v)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(byte_addr, __float_as_int(v)));
+}
+
+// Lattice schemes of vnoise: per-lane hashing of all eight corners; scalar
+// hashing when the whole wave sits in one cell; or the lane window below.
+enum { kNoisePerLane = 0, kNoiseWaveCell = 1, kNoiseLaneWindow = 2 };
+
+// Lane window (whole wave active): the wave's cells lie within +-1 cell of
+// lane 32's cell in each axis in practice, so the 4x4x4 lattice corners
+// around that cell are exactly 64 values.  Lane k hashes corner
+// (bx + (k & 3), by + (k >> 2 & 3), bz + (k >> 4)) once, and every lane
+// reads its eight corners from the lanes that hashed them.  One hash per lane
+// per octave instead of eight; the values are lattice()'s (coordinates wrap
+// in 32 bits, as lattice()'s multiplications do).  Returns false (nothing
+// read) when a lane's cell falls outside the window.
+MP_DEV bool window_corners(int32_t ix, int32_t iy, int32_t iz, uint32_t seed, float c[8])
+{
+    const uint32_t bx = (uint32_t)__builtin_amdgcn_readlane(ix, 32) - 1u;
+    const uint32_t by = (uint32_t)__builtin_amdgcn_readlane(iy, 32) - 1u;
+    const uint32_t bz = (uint32_t)__builtin_amdgcn_readlane(iz, 32) - 1u;
+    const uint32_t dx = (uint32_t)ix - bx, dy = (uint32_t)iy - by, dz = (uint32_t)iz - bz;
+    if (!__all(dx <= 2u && dy <= 2u && dz <= 2u)) return false;
+    const uint32_t k = threadIdx.x & 63u;
+    const float hv = lattice((int32_t)(bx + (k & 3u)), (int32_t)(by + ((k >> 2) & 3u)),
+                             (int32_t)(bz + (k >> 4)), seed);
+    const int a = (int)((dx + 4u * dy + 16u * dz) << 2);
+    c[0] = lane_value(a, hv);      c[1] = lane_value(a + 4, hv);    // z0 y0: x0 x1
+    c[2] = lane_value(a + 16, hv); c[3] = lane_value(a + 20, hv);   // z0 y1
+    c[4] = lane_value(a + 64, hv); c[5] = lane_value(a + 68, hv);   // z1 y0
+    c[6] = lane_value(a + 80, hv); c[7] = lane_value(a + 84, hv);   // z1 y1
+    return true;
+}
+
+MP_DEV float vnoise(float x, float y, float z, uint32_t seed, int scheme = kNoisePerLane)
 {
     float fx = floorf(x), fy = floorf(y), fz = floorf(z);
     int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
@@ -206,7 +242,11 @@ MP_DEV float vnoise(float x, float y, float z, uint32_t seed, bool wave_cells = 
     const int32_t jx = __builtin_amdgcn_readfirstlane(ix);
     const int32_t jy = __builtin_amdgcn_readfirstlane(iy);
     const int32_t jz = __builtin_amdgcn_readfirstlane(iz);
-    if (wave_cells && __all(ix == jx && iy == jy && iz == jz)) {
+    float c[8];
+    if (scheme == kNoiseLaneWindow && window_corners(ix, iy, iz, seed, c)) {
+        c000 = c[0]; c100 = c[1]; c010 = c[2]; c110 = c[3];
+        c001 = c[4]; c101 = c[5]; c011 = c[6]; c111 = c[7];
+    } else if (scheme == kNoiseWaveCell && __all(ix == jx && iy == jy && iz == jz)) {
         // every active lane is in the same lattice cell: the eight corner
         // hashes are wave-uniform (scalar unit), the values are identical
         c000 = lattice_wave(jx, jy, jz, seed);         c100 = lattice_wave(jx + 1, jy, jz, seed);
@@ -225,12 +265,15 @@ MP_DEV float vnoise(float x, float y, float z, uint32_t seed, bool wave_cells = 
     return lerpf_(y0, y1, wz);
 }
 
-MP_DEV float noise3(const DevParams& prm, float x, float y, float z)
+// full_wave: every lane of the wave runs this call (the lane window needs
+// all 64 lanes); otherwise the window scheme falls back to the scalar one.
+MP_DEV float noise3(const DevParams& prm, float x, float y, float z, bool full_wave = false)
 {
     float acc = 0.0f, amp = 1.0f, norm = 0.0f, freq = prm.disp_frequency;
-    const bool wave_cells = prm.noise_wave_cells != 0;
+    const int scheme = prm.noise_wave_cells == kNoiseLaneWindow && !full_wave ? kNoiseWaveCell
+                                                                              : prm.noise_wave_cells;
     for (int k = 0; k < prm.disp_octaves; ++k) {
-        acc = acc + amp * vnoise(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, wave_cells);
+        acc = acc + amp * vnoise(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, scheme);
         norm = norm + amp;
         amp = amp * 0.5f;
         freq = freq * 2.0f;
@@ -345,12 +388,13 @@ MP_DEV void displaced_point(const float4* __restrict__ cp, int type, float u, fl
 // Same with the normal's column sums precomputed (col: 8 float4 of the
 // point's u column).
 MP_DEV void displaced_point_cols(const float4* __restrict__ cp, const float4* __restrict__ col,
-                                 float u, float v, const DevParams& prm, float amp, float out[4])
+                                 float u, float v, const DevParams& prm, float amp, float out[4],
+                                 bool full_wave = false)
 {
     eval_bezier(cp, u, v, out);
     float h[3];
     normal_from_columns(col, v, h);
-    float d = amp * noise3(prm, out[0], out[1], out[2]);
+    float d = amp * noise3(prm, out[0], out[1], out[2], full_wave);
     out[0] = out[0] + d * h[0];
     out[1] = out[1] + d * h[1];
     out[2] = out[2] + d * h[2];

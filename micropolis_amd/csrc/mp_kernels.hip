@@ -781,22 +781,28 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
             __syncthreads();
         }
         if (MODE != 2 || ob.type != MP_TYPE_BEZIER) {
-            for (int g = tid; g < G; g += Q) {
+            // wave-uniform trip count with every lane active (the lane-window
+            // noise exchanges values across the wave); lanes past the last
+            // point repeat it and store nothing
+            for (int gb = wave * 64; gb < G; gb += Q) {
+                const int g = min(gb + lane, G - 1);
                 const int nu = g % G1, nv = g / G1;
                 const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
                 const float uu = rec.umin + (rec.umax - rec.umin) * tu;
                 const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
                 float P[4], e[4], q[4];
-                if (disp_cols) displaced_point_cols(s_cp, s_col + 8 * nu, uu, vv, prm, ob.disp_amplitude, P);
+                if (disp_cols) displaced_point_cols(s_cp, s_col + 8 * nu, uu, vv, prm, ob.disp_amplitude, P, true);
                 else if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
                 else eval_patch(s_cp, ob.type, uu, vv, P);
                 mat4_mul(ob.mv, P, e);
                 mat4_mul(vw.proj, e, q);
-                s_pos[g] = make_float4(e[0], e[1], e[2], e[3]);
                 const int32_t px = f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2));
                 const int32_t py = f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2));
-                s_pxl[g] = make_int2(px, py);
-                s_dep[g] = q[2] / q[3];
+                if (gb + lane < G) {
+                    s_pos[g] = make_float4(e[0], e[1], e[2], e[3]);
+                    s_pxl[g] = make_int2(px, py);
+                    s_dep[g] = q[2] / q[3];
+                }
             }
         } else {
             // displaced: surface point and normal, then the value noise with
@@ -1082,6 +1088,7 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
     const DevView& vw = views[blockIdx.y];
     const uint32_t n = (uint32_t)vw.grid_w * (uint32_t)vw.grid_h * 64u;
     const float ac[4] = { 0.015f, 0.015f, 0.015f, 1.0f };
+    const uint32_t qshift = (uint32_t)__builtin_ctz((unsigned)Q);   // Q = N*N is a power of two
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const unsigned long long key = gptr(vw.keys)[i];
         gptr(vw.keys)[i] = MP_EMPTY_KEY;
@@ -1099,7 +1106,7 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
         } else {
             id = (uint32_t)key;
             d = unmono_bits((uint32_t)(key >> 32));
-            const uint32_t rr = vw.range_base + id / (uint32_t)Q;
+            const uint32_t rr = vw.range_base + (id >> qshift);
             const float s = quad_store[(size_t)vw.range_base * Q + id];
             const DevObject& ob = objs[canon[rr].obj_depth & 0xffffffu];
 #pragma unroll
@@ -1119,8 +1126,10 @@ hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           const RangeRec* canon, const DevObject* objs, const float* quad_store,
                           int patch_size, int write_rgba32, hipStream_t s)
 {
+    // one key per thread: every pixel's gather chain (range, object,
+    // Lambert term) is in flight at once
     int bx = (max_pixels + 255) / 256;
-    if (bx > 2048) bx = 2048;
+    if (bx > 65535) bx = 65535;
     if (bx < 1) bx = 1;
     hipLaunchKernelGGL(k_resolve, dim3(bx, n_views), dim3(256), 0, s, views, canon, objs,
                        quad_store, patch_size * patch_size, write_rgba32);
