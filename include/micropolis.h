@@ -235,6 +235,10 @@ typedef struct mp_tuning {
                                 range count, > 0 dynamic chunks of that size */
     int32_t split_levels;    /* split_engine 1: level launches before the
                                 persistent tail (0 = as many as the frame needs) */
+    int32_t split_budget;    /* LOCAL, split_engine 0, no work sharing: steps
+                                a persistent group runs before its queued and
+                                stacked ranges are dealt to all groups for a
+                                second launch (0 = one launch) */
     double  frame_ms;        /* mp_autotune: replay time of the chosen setting */
 } mp_tuning;
 int mp_get_tuning(mp_context* ctx, mp_tuning* tuning);
@@ -243,8 +247,9 @@ int mp_set_tuning(mp_context* ctx, const mp_tuning* tuning);
 /* Autotuner: replay the last finished LOCAL/BALANCED frame or batch under
  * candidate settings, `replays` graph replays each (best of two rounds):
  * first the raster schedule (work-groups per CU x static / dynamic ranges),
- * then the split (persistent with / without work sharing, or a number of
- * level launches before the persistent tail); keep the fastest. */
+ * then the split (persistent with / without work sharing or a step budget,
+ * or a number of level launches before the persistent tail); keep the
+ * fastest. */
 int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen);
 
 /* Replay the last LOCAL/BALANCED frame n times with HIP events between its

@@ -119,6 +119,8 @@ struct mp_context {
     int raster_grid_size = 0;
     int split_groups = 0;
     int stack_cap = 0;
+    int split_spread = 1;        // persistent split: one sample per lane on narrow steps
+    int split_budget = 8;        // LOCAL persistent split: steps before rebalancing (0: none)
     mp_stats stats;
     std::vector<int32_t> pass_counts;
     bool frame_valid = false;
@@ -353,6 +355,8 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     if (const char* e = getenv("MP_SPILL_AT")) ctx->spill_at = atoi(e);
     if (const char* e = getenv("MP_LINGER")) ctx->linger = atoi(e);
     if (const char* e = getenv("MP_STACK_CAP")) ctx->stack_cap = std::max(256, atoi(e));
+    if (const char* e = getenv("MP_SPLIT_SPREAD")) ctx->split_spread = atoi(e);
+    if (const char* e = getenv("MP_SPLIT_BUDGET")) ctx->split_budget = std::max(0, atoi(e));
     if (const char* e = getenv("MP_SPLIT_ENGINE")) ctx->split_engine = atoi(e);
     if (const char* e = getenv("MP_CPU_THREADS")) ctx->cpu_threads = atoi(e);
     if (const char* e = getenv("MP_LEVEL_LANES")) ctx->level_lanes = atoi(e) == 1 ? 1 : 9;
@@ -688,16 +692,24 @@ static int upload_tables(mp_context* ctx)
     return MP_OK;
 }
 
+// LOCAL persistent split in two launches: a step budget, then the left
+// ranges dealt to all groups
+static bool rebalancing(const mp_context* ctx)
+{
+    return ctx->split_engine == 0 && ctx->split_budget > 0 &&
+           ctx->cfg.bound_n_split_method != MP_BNS_BALANCED && !ctx->work_sharing;
+}
+
 static int ensure_queues(mp_context* ctx)
 {
     int rc;
     const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
     int groups = ctx->cfg.local_bns_work_groups;
     if (groups <= 0) {
-        groups = ctx->num_cus * split_persistent_occupancy((size_t)ctx->stack_cap * sizeof(RangeRec));
+        groups = ctx->num_cus * split_persistent_occupancy(split_persistent_lds(ctx->stack_cap));
     }
     ctx->split_groups = groups;
-    if (ctx->split_engine == 1) {
+    if (ctx->split_engine == 1 || rebalancing(ctx)) {
         // level lists: sized for the widest level; overflow re-renders bigger
         const uint32_t want = std::max<uint32_t>(ctx->level_cap, std::max<uint32_t>(65536, 8 * ctx->total_roots));
         if (want > ctx->level_cap || !ctx->d_level_a) {
@@ -711,7 +723,7 @@ static int ensure_queues(mp_context* ctx)
             HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_level_cnt, 64 * sizeof(uint32_t), 0));
         }
     }
-    const size_t roots_or_left = std::max<size_t>(ctx->total_roots, ctx->split_engine == 1 ? ctx->level_cap : 0);
+    const size_t roots_or_left = std::max<size_t>(ctx->total_roots, ctx->split_engine == 1 || rebalancing(ctx) ? ctx->level_cap : 0);
     const size_t stride = balanced && ctx->split_engine != 1 ? 1 : (roots_or_left + groups - 1) / groups;
     const size_t entries = std::max<size_t>(1, stride * groups);
     if (entries > ctx->queue_entries) {
@@ -759,7 +771,7 @@ static SplitQueues split_queues(mp_context* ctx)
     const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
     q.queue = ctx->d_queue;
     q.queue_cnt = ctx->d_queue_cnt;
-    const size_t left = std::max<size_t>(ctx->total_roots, ctx->split_engine == 1 ? ctx->level_cap : 0);
+    const size_t left = std::max<size_t>(ctx->total_roots, ctx->split_engine == 1 || rebalancing(ctx) ? ctx->level_cap : 0);
     q.stride = (balanced && ctx->split_engine != 1) ? 0 : (int64_t)((left + ctx->split_groups - 1) / ctx->split_groups);
     q.shared_q = ctx->d_shared_q;
     q.shared_cnt = ctx->d_shared_cnt;
@@ -773,6 +785,11 @@ static SplitQueues split_queues(mp_context* ctx)
     q.linger = ctx->linger >= 0 ? ctx->linger : ctx->num_cus;
     q.share = ctx->d_share;
     q.pending = nullptr;
+    q.spread = ctx->split_spread;
+    q.step_budget = 0;
+    q.left = nullptr;
+    q.left_cnt = nullptr;
+    q.left_cap = 0;
     return q;
 }
 
@@ -810,6 +827,17 @@ static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
                                         ctx->total_roots, ctx->d_level_a, ctx->d_level_b, ctx->level_cap,
                                         ctx->d_level_cnt, ctx->level_launches, q, canon_buffers(ctx), ctx->d_fc,
                                         ctx->level_grid_size, ctx->level_lanes, ctx->stream));
+    } else if (ctx->total_roots && rebalancing(ctx)) {
+        q.step_budget = ctx->split_budget;
+        q.left = ctx->d_level_a;
+        q.left_cnt = ctx->d_level_cnt + 63;
+        q.left_cap = ctx->level_cap;
+        HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
+                                       balanced, ctx->stream));
+        HIPCHK(ctx, launch_split_persistent(ctx->d_objs, ctx->d_views, ctx->prm, q,
+                                            canon_buffers(ctx), ctx->d_fc, balanced, ctx->stream));
+        HIPCHK(ctx, launch_split_rebalanced(ctx->d_objs, ctx->d_views, ctx->prm, q, canon_buffers(ctx),
+                                            ctx->d_fc, ctx->stream));
     } else if (ctx->total_roots) {
         HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
                                        balanced, ctx->stream));
@@ -1377,6 +1405,7 @@ extern "C" int mp_get_tuning(mp_context* ctx, mp_tuning* t)
     t->raster_per_cu = ctx->raster_grid_size / std::max(1, ctx->num_cus);
     t->raster_chunk = ctx->prm.raster_chunk;
     t->split_levels = ctx->level_limit;
+    t->split_budget = ctx->split_budget;
     t->frame_ms = 0.0;
     return MP_OK;
 }
@@ -1390,6 +1419,7 @@ static int apply_tuning(mp_context* ctx, const mp_tuning& t)
     ctx->raster_grid_size = ctx->num_cus * std::max(1, t.raster_per_cu);
     ctx->prm.raster_chunk = t.raster_chunk;
     ctx->level_limit = std::max(0, t.split_levels);
+    ctx->split_budget = std::max(0, t.split_budget);
     ctx->level_grid_size = 0;
     if (ctx->split_engine == 1)
         ctx->level_launches = ctx->level_limit > 0 ? std::min(ctx->level_limit, ctx->cfg.max_split_depth)
@@ -1410,7 +1440,8 @@ extern "C" int mp_set_tuning(mp_context* ctx, const mp_tuning* t)
 {
     if (!ctx || !t) return MP_E_INVALID;
     if (t->split_engine < 0 || t->split_engine > 1 || t->raster_per_cu < 0 || t->raster_per_cu > 64 ||
-        t->raster_chunk < -1 || t->raster_chunk > 4096 || t->split_levels < 0 || t->split_levels > 23)
+        t->raster_chunk < -1 || t->raster_chunk > 4096 || t->split_levels < 0 || t->split_levels > 23 ||
+        t->split_budget < 0 || t->split_budget > 4096)
         return fail(ctx, MP_E_INVALID, "tuning out of range");
     mp_tuning u = *t;
     if (u.raster_per_cu == 0) u.raster_per_cu = raster_grid(ctx->cfg.reyes_patch_size, ctx->num_cus, ctx->prm) / ctx->num_cus;
@@ -1461,17 +1492,22 @@ extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
     // stage 2: split schedule
     cand.clear();
     for (int ws : { 0, 1 }) {           // persistent split
-        mp_tuning c = cur;
-        c.split_engine = 0;
-        c.work_sharing = ws;
-        c.split_levels = 0;
-        cand.push_back(c);
+        for (int budget : { 0, 5, 8 }) {    // (a budget applies to LOCAL frames without sharing)
+            if (budget && (ws || ctx->cfg.bound_n_split_method == MP_BNS_BALANCED)) continue;
+            mp_tuning c = cur;
+            c.split_engine = 0;
+            c.work_sharing = ws;
+            c.split_levels = 0;
+            c.split_budget = budget;
+            cand.push_back(c);
+        }
     }
     for (int lv : { 0, 3, 5 }) {         // level launches, then the persistent tail
         mp_tuning c = cur;
         c.split_engine = 1;
         c.work_sharing = 0;
         c.split_levels = lv;
+        c.split_budget = 0;
         cand.push_back(c);
     }
     b = pick_fastest(ctx, cand, replays);

@@ -30,6 +30,14 @@ struct SplitQueues {
     int32_t linger;          // idle groups wait (bounded) for spilled chunks
     ShareState* share;
     const uint32_t* pending; // if set and zero on entry, there is no work: exit
+    int32_t spread;          // narrow steps: one sample per lane over the group's waves
+    // step budget: a group that has run step_budget steps (0: no budget)
+    // appends its queued and stacked ranges to `left` and exits; a second
+    // launch deals them round-robin to all groups
+    int32_t step_budget;
+    RangeRec* left;
+    uint32_t* left_cnt;
+    uint32_t left_cap;
 };
 
 struct CanonBuffers {
@@ -93,7 +101,13 @@ hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView*
                                uint32_t total_roots, RangeRec* list_a, RangeRec* list_b, uint32_t list_cap,
                                uint32_t* level_cnt, int n_levels, const SplitQueues& q,
                                const CanonBuffers& cb, FrameCounters* fc, int grid, int lanes_per_range, hipStream_t s);
+// After a budgeted persistent split (q.step_budget > 0): deal q.left to the
+// groups' private queues and finish the split without a budget.
+hipError_t launch_split_rebalanced(const DevObject* objs, const DevView* views, const DevParams& prm,
+                                   const SplitQueues& q, const CanonBuffers& cb, FrameCounters* fc,
+                                   hipStream_t s);
 hipError_t launch_xyz_to_cps(const float* xyz, float4* cps, size_t n, hipStream_t s);
 int split_persistent_occupancy(size_t lds_bytes);
+size_t split_persistent_lds(int stack_cap);   // LDS bytes of one persistent split group
 
 }  // namespace mp

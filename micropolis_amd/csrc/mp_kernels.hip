@@ -1,7 +1,8 @@
 // HIP kernels of the REYES hot path for gfx950 (CDNA4, wave64).
 //
-//  split      k_split_persistent   LOCAL / BALANCED: persistent one-wave
-//                                  work-groups, LDS range stack, wave-ballot
+//  split      k_split_persistent   LOCAL / BALANCED: persistent work-groups
+//                                  (controller wave + sample waves), LDS
+//                                  range stack, wave-ballot
 //                                  compaction (kernels/bound_n_split_local.cl,
 //                                  kernels/bound_n_split_balanced.cl)
 //             k_bound_segment +    BREADTH / BOUNDED: one level or batch,
@@ -115,6 +116,7 @@ __global__ void k_init_queues(const DevObject* __restrict__ objs, int n_objs, ui
     if (g < (uint32_t)q.n_groups)
         q.queue_cnt[g] = balanced ? 0 : (int32_t)((total + (uint32_t)(q.n_groups - 1) - g) / (uint32_t)q.n_groups);
     if (balanced && g == 0) *q.shared_cnt = (int32_t)total;
+    if (g == 0 && q.left_cnt) *q.left_cnt = 0u;
     if (g >= total) return;
     int lo = 0, hi = n_objs - 1;
     while (lo < hi) {
@@ -152,6 +154,117 @@ MP_DEV void wave_count_add(uint32_t* counts, uint32_t key, bool pred)
     }
 }
 
+// Orders one wave's LDS accesses across its lanes (a wave's LDS operations
+// complete in issue order; this keeps the compiler from moving them).
+MP_DEV void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Persistent split work-group: one controller wave (queues, stack, draws)
+// and kSplitWaves - 1 sample waves; wave w evaluates column iu = w of
+// bound()'s 3x3 samples.  A step of at most kNarrowStep ranges gives each
+// range three lanes per wave (one sample per lane: one patch evaluation of
+// latency); a wider step gives each range one lane per wave (three samples
+// per lane) instead of nine samples on one lane.
+constexpr int kSplitWaves = 3;
+constexpr int kNarrowStep = 21;
+
+// Sample (iu, iv) of bound()'s samples of rec: eye-space x, y, z, screen x, y.
+MP_DEV void bound_one(const DevObject* __restrict__ objs, const DevView* __restrict__ views,
+                      const RangeRec& rec, int iu, int iv, float (&v)[5])
+{
+    const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
+    const DevView& P = views[ob.view];
+    float e[3];
+    if (ob.type == MP_TYPE_GREGORY) {
+        const float4* cp = ob.cps + (size_t)rec.pid * 20;
+        float4 C[20];
+#pragma unroll
+        for (int k = 0; k < 20; ++k) C[k] = ldg4(cp + k);
+        bound_sample<20>(C, iu, iv, rec.umin, rec.vmin, rec.umax, rec.vmax, ob.mv, P, e, v[3], v[4]);
+    } else {
+        const float4* cp = ob.cps + (size_t)rec.pid * 16;
+        float4 C[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) C[k] = ldg4(cp + k);
+        bound_sample<16>(C, iu, iv, rec.umin, rec.vmin, rec.umax, rec.vmax, ob.mv, P, e, v[3], v[4]);
+    }
+    v[0] = e[0]; v[1] = e[1]; v[2] = e[2];
+}
+
+// Column iu (samples iu*3 + 0..2) of bound()'s samples of rec.
+MP_DEV void bound_column(const DevObject* __restrict__ objs, const DevView* __restrict__ views,
+                         const RangeRec& rec, int iu, float (&v)[15])
+{
+    const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
+    const DevView& P = views[ob.view];
+    float e[3];
+    if (ob.type == MP_TYPE_GREGORY) {
+        const float4* cp = ob.cps + (size_t)rec.pid * 20;
+        float4 C[20];
+#pragma unroll
+        for (int k = 0; k < 20; ++k) C[k] = ldg4(cp + k);
+#pragma unroll
+        for (int iv = 0; iv < 3; ++iv) {
+            bound_sample<20>(C, iu, iv, rec.umin, rec.vmin, rec.umax, rec.vmax, ob.mv, P, e, v[9 + iv], v[12 + iv]);
+            v[iv] = e[0]; v[3 + iv] = e[1]; v[6 + iv] = e[2];
+        }
+    } else {
+        const float4* cp = ob.cps + (size_t)rec.pid * 16;
+        float4 C[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) C[k] = ldg4(cp + k);
+#pragma unroll
+        for (int iv = 0; iv < 3; ++iv) {
+            bound_sample<16>(C, iu, iv, rec.umin, rec.vmin, rec.umax, rec.vmax, ob.mv, P, e, v[9 + iv], v[12 + iv]);
+            v[iv] = e[0]; v[3 + iv] = e[1]; v[6 + iv] = e[2];
+        }
+    }
+}
+
+// DRAW ranges a split work-group buffers in LDS before appending them to the
+// frame's draw list: the list's counter is one address that every
+// work-group's returning atomic serialises on, so a group appends once per
+// kDrawBuf draws (usually once, at its end) instead of once per step.
+constexpr int kDrawBuf = 32;
+
+// Writes the buffered draws (lanes < dfill) and this step's draws (lanes
+// with is_draw, at slot among nd) to the draw list under one counter atomic
+// and counts them per patch.  Returns false when the list overflows.
+MP_DEV bool flush_draws(const DevObject* __restrict__ objs, const CanonBuffers& cb,
+                        FrameCounters* fc, const RangeRec* dbuf, int dfill,
+                        const RangeRec& rec, bool is_draw, uint32_t slot, int nd)
+{
+    const int lane = threadIdx.x & 63;
+    wave_sync();   // the buffer's LDS writes by other lanes
+    uint32_t dbase = 0;
+    if (lane == 0) dbase = atomicAdd(&fc->draw_count, (uint32_t)(dfill + nd));
+    dbase = __shfl(dbase, 0);
+    {
+        const bool have = lane < dfill;
+        RangeRec b;
+        if (have) b = dbuf[lane];
+        const uint32_t idx = dbase + (uint32_t)lane;
+        const bool put = have && idx < cb.cap;
+        if (put) cb.draw[idx] = b;
+        wave_count_add(cb.counts, put ? gpid_of(objs, b) : 0u, put);
+    }
+    if (nd) {
+        const uint32_t idx = dbase + (uint32_t)dfill + slot;
+        const bool put = is_draw && idx < cb.cap;
+        if (put) cb.draw[idx] = rec;
+        wave_count_add(cb.counts, put ? gpid_of(objs, rec) : 0u, put);
+    }
+    if (dbase + (uint32_t)(dfill + nd) > cb.cap) {
+        if (lane == 0) atomicOr(&fc->overflow, 1u);
+        return false;
+    }
+    return true;
+}
+
 constexpr uint32_t kFlagSpin = 1u << 20;     // bound on waiting for a reserved chunk
 constexpr uint32_t kLingerPolls = 1u << 14;  // bound on waiting for spilled work
 
@@ -180,26 +293,62 @@ MP_DEV int try_pop_chunk(const SplitQueues& q, uint32_t epoch, FrameCounters* fc
     }
 }
 
-// One wave per work-group, following kernels/bound_n_split_local.cl: pop up
-// to 64 ranges (private queue, then the shared BALANCED queue, then spilled
-// chunks, then the LDS stack), bound them, push split children on the LDS
-// ring stack (ballot rank) and append DRAW ranges to the frame's draw list
-// (one atomic per wave).  A stack that grows past spill_at publishes its 64
-// oldest (shallowest) ranges as a chunk that idle work-groups pick up.  A
-// work-group with no work keeps polling for chunks while others are active
-// (bounded); every chunk is popped because its publisher drains the chunk
-// queue before it exits.
-__global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __restrict__ objs,
-                                                         const DevView* __restrict__ views,
-                                                         DevParams prm, SplitQueues q,
-                                                         CanonBuffers cb, FrameCounters* fc,
-                                                         int balanced)
+// Following kernels/bound_n_split_local.cl, the controller wave pops up to
+// 64 ranges (private queue, then the shared BALANCED queue, then spilled
+// chunks, then the LDS stack), bounds them (with the sample waves, or with
+// 3 or 9 of its own lanes per range on narrow steps), pushes split children
+// on the LDS ring stack (ballot rank) and appends DRAW ranges to the frame's
+// draw list (buffered in LDS, one atomic per kDrawBuf draws).  A stack that
+// grows past spill_at publishes its 64 oldest (shallowest) ranges as a chunk
+// that idle work-groups pick up.  A work-group with no work keeps polling
+// for chunks while others are active (bounded); every chunk is popped
+// because its publisher drains the chunk queue before it exits.  The sample
+// waves wait at the work-group barrier between the controller's requests;
+// a negative request count releases them.
+__global__ __launch_bounds__(64 * kSplitWaves) void k_split_persistent(const DevObject* __restrict__ objs,
+                                                                       const DevView* __restrict__ views,
+                                                                       DevParams prm, SplitQueues q,
+                                                                       CanonBuffers cb, FrameCounters* fc,
+                                                                       int balanced)
 {
-    extern __shared__ RangeRec ring[];
-    const int lane = threadIdx.x;
+    extern __shared__ RangeRec ring[];   // stack_cap stack entries, then kDrawBuf draws
+    __shared__ RangeRec s_stage[64];                  // ranges of a sampled step
+    __shared__ float s_xch[kSplitWaves - 1][15][64];  // sample waves' columns
+    __shared__ int s_cmd;                             // ranges in the step; < 0: exit
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int wid = blockIdx.x;
-    if (q.pending && *q.pending == 0u) return;   // nothing left by the level launches
+    if (q.pending && *q.pending == 0u) return;   // nothing left by the level launches (whole group)
+    if (wave > 0) {
+        for (;;) {
+            __syncthreads();   // request posted
+            const int cmd = s_cmd;
+            if (cmd < 0) return;
+            if (cmd <= kNarrowStep) {
+                const int g = lane / 3;
+                const RangeRec rec = s_stage[min(g, 63)];
+                float v[5] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+                if (g < cmd && (int)(rec.obj_depth >> 24) < prm.max_split_depth - 1)
+                    bound_one(objs, views, rec, wave, lane - 3 * g, v);
+#pragma unroll
+                for (int t = 0; t < 5; ++t) s_xch[wave - 1][t][lane] = v[t];
+            } else {
+                const RangeRec rec = s_stage[lane];
+                float v[15];
+#pragma unroll
+                for (int t = 0; t < 15; ++t) v[t] = 0.0f;
+                if (lane < cmd && (int)(rec.obj_depth >> 24) < prm.max_split_depth - 1)
+                    bound_column(objs, views, rec, wave, v);
+#pragma unroll
+                for (int t = 0; t < 15; ++t) s_xch[wave - 1][t][lane] = v[t];
+            }
+            __syncthreads();   // samples written
+        }
+    }
     const int cap = q.stack_cap;
+    RangeRec* dbuf = ring + cap;
+    int dfill = 0;
+    bool ok = true;
     RangeRec* myq = q.queue + (int64_t)wid * q.stride;
     int base = 0, sh = 0;
     int qcnt = q.queue_cnt[wid];
@@ -208,8 +357,30 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
     const bool sharing = q.chunk_cap > 0;
     if (sharing && lane == 0) atomicAdd(&q.share->active, 1u);
     bool active = true;
+    int steps = 0;
 
     for (;;) {
+        if (q.step_budget > 0 && steps == q.step_budget) {
+            // budget spent: hand the queued and stacked ranges to the next launch
+            const int total = qcnt + sh;
+            uint32_t lb = 0;
+            if (lane == 0 && total) lb = atomicAdd(q.left_cnt, (uint32_t)total);
+            lb = __shfl(lb, 0);
+            for (int i = lane; i < total; i += 64) {
+                RangeRec r;
+                if (i < qcnt) {
+                    r = myq[i];
+                } else {
+                    int k = base + i - qcnt;
+                    if (k >= cap) k -= cap;
+                    r = ring[k];
+                }
+                if (lb + (uint32_t)i < q.left_cap) q.left[lb + (uint32_t)i] = r;
+            }
+            if (lane == 0 && lb + (uint32_t)total > q.left_cap) atomicOr(&fc->overflow, 8u);
+            break;
+        }
+        ++steps;
         int cnt = 0, start = 0, scnt = 0, src = 0;
         if (sh < 64) {
             const int want = 64 - sh;
@@ -277,6 +448,7 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
         }
 
         RangeRec rec;
+        rec.pid = 0; rec.obj_depth = 0; rec.umin = rec.vmin = rec.umax = rec.vmax = 0.0f;
         bool occ = false;
         if (lane < cnt) {
             rec = (src == 2) ? q.chunks[start + lane] : (src == 1 ? q.shared_q[start + lane] : myq[start + lane]);
@@ -286,10 +458,79 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
             rec = ring[k >= cap ? k - cap : k];
             occ = true;
         }
-        __syncthreads();
+        wave_sync();   // stack reads before this step's pushes
         int flags = 0;
-        if (occ) flags = bound_rec(objs, views, prm, rec);
-        nbounds += (uint32_t)(cnt + scnt);
+        const int n = cnt + scnt;
+        if (q.spread && n <= kNarrowStep) {
+            // narrow step (the deep tail of a split tree): nine lanes per
+            // range over the three waves, one sample each
+            s_stage[lane] = rec;
+            if (lane == 0) s_cmd = n;
+            __syncthreads();   // request posted
+            const int g = lane / 3, iv = lane - 3 * g;
+            const RangeRec rg = s_stage[min(g, 63)];
+            const bool valid = g < n;
+            const bool live = valid && (int)(rg.obj_depth >> 24) < prm.max_split_depth - 1;
+            float v[5] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+            if (live) bound_one(objs, views, rg, 0, iv, v);
+            __syncthreads();   // samples written
+            // sample (iu, iv) of range g: wave iu, lane 3g + iv
+            float ex[9], ey[9], ez[9], sx[9], sy[9];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int src = min(3 * g + j, 63);
+                ex[j] = __shfl(v[0], src); ey[j] = __shfl(v[1], src); ez[j] = __shfl(v[2], src);
+                sx[j] = __shfl(v[3], src); sy[j] = __shfl(v[4], src);
+#pragma unroll
+                for (int w = 1; w < kSplitWaves; ++w) {
+                    const int k = w * 3 + j;
+                    ex[k] = s_xch[w - 1][0][src]; ey[k] = s_xch[w - 1][1][src];
+                    ez[k] = s_xch[w - 1][2][src]; sx[k] = s_xch[w - 1][3][src];
+                    sy[k] = s_xch[w - 1][4][src];
+                }
+            }
+            rec = rg;
+            occ = valid && iv == 0;
+            if (occ && live) {
+                const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
+                flags = bound_decide(ex, ey, ez, sx, sy, rec.umin, rec.vmin, rec.umax, rec.vmax,
+                                     views[ob.view], prm, ob.split_limit);
+            }
+        } else {
+            // one sample column per wave
+            s_stage[lane] = rec;
+            if (lane == 0) s_cmd = n;
+            __syncthreads();   // request posted
+            const bool live = occ && (int)(rec.obj_depth >> 24) < prm.max_split_depth - 1;
+            float c0[15];
+#pragma unroll
+            for (int t = 0; t < 15; ++t) c0[t] = 0.0f;
+            if (live) bound_column(objs, views, rec, 0, c0);
+            __syncthreads();   // samples written
+            if (live) {
+                float ex[9], ey[9], ez[9], sx[9], sy[9];
+#pragma unroll
+                for (int iv = 0; iv < 3; ++iv) {
+                    ex[iv] = c0[iv]; ey[iv] = c0[3 + iv]; ez[iv] = c0[6 + iv];
+                    sx[iv] = c0[9 + iv]; sy[iv] = c0[12 + iv];
+                }
+#pragma unroll
+                for (int w = 1; w < kSplitWaves; ++w)
+#pragma unroll
+                    for (int iv = 0; iv < 3; ++iv) {
+                        const int k = w * 3 + iv;
+                        ex[k] = s_xch[w - 1][iv][lane];
+                        ey[k] = s_xch[w - 1][3 + iv][lane];
+                        ez[k] = s_xch[w - 1][6 + iv][lane];
+                        sx[k] = s_xch[w - 1][9 + iv][lane];
+                        sy[k] = s_xch[w - 1][12 + iv][lane];
+                    }
+                const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
+                flags = bound_decide(ex, ey, ez, sx, sy, rec.umin, rec.vmin, rec.umax, rec.vmax,
+                                     views[ob.view], prm, ob.split_limit);
+            }
+        }
+        nbounds += (uint32_t)n;
 
         const uint64_t sb = __ballot(flags & 2);
         const int ns = __popcll(sb);
@@ -314,19 +555,17 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
         const uint64_t db = __ballot(flags & 1);
         const int nd = __popcll(db);
         if (nd) {
-            uint32_t dbase = 0;
-            if (lane == 0) dbase = atomicAdd(&fc->draw_count, (uint32_t)nd);
-            dbase = __shfl(dbase, 0);
-            const uint32_t idx = dbase + lanes_below(db);
-            const bool put = (flags & 1) && idx < cb.cap;
-            if (put) cb.draw[idx] = rec;
-            wave_count_add(cb.counts, put ? gpid_of(objs, rec) : 0u, put);
-            if (dbase + (uint32_t)nd > cb.cap) {
-                if (lane == 0) atomicOr(&fc->overflow, 1u);
-                break;
+            const uint32_t slot = lanes_below(db);
+            if (dfill + nd <= kDrawBuf) {
+                if (flags & 1) dbuf[dfill + (int)slot] = rec;
+                dfill += nd;
+            } else {
+                ok = flush_draws(objs, cb, fc, dbuf, dfill, rec, (flags & 1) != 0, slot, nd);
+                dfill = 0;
+                if (!ok) break;
             }
         }
-        __syncthreads();
+        wave_sync();
 
         // publish the 64 oldest stack entries while the stack is deep
         while (sharing && sh > q.spill_at) {
@@ -339,13 +578,19 @@ __global__ __launch_bounds__(64) void k_split_persistent(const DevObject* __rest
             q.chunks[(size_t)idx * 64 + lane] = ring[k];
             if (lane == 0) q.chunk_count[idx] = 64u;
             __threadfence();
-            __syncthreads();
+            wave_sync();
             if (lane == 0) __hip_atomic_store(&q.chunk_ready[idx], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             base += 64;
             if (base >= cap) base -= cap;
             sh -= 64;
         }
     }
+    if (ok && dfill > 0) {
+        RangeRec none;
+        flush_draws(objs, cb, fc, dbuf, dfill, none, false, 0u, 0);
+    }
+    if (lane == 0) s_cmd = -1;
+    __syncthreads();   // release the sample waves
     if (lane == 0) {
         if (sharing && active) atomicSub(&q.share->active, 1u);
         atomicAdd(&fc->bounds, (unsigned long long)nbounds);
@@ -358,8 +603,8 @@ hipError_t launch_split_persistent(const DevObject* objs, const DevView* views,
                                    const CanonBuffers& cb, FrameCounters* fc, int balanced,
                                    hipStream_t s)
 {
-    size_t lds = (size_t)q.stack_cap * sizeof(RangeRec);
-    hipLaunchKernelGGL(k_split_persistent, dim3(q.n_groups), dim3(64), lds, s, objs, views, prm,
+    size_t lds = split_persistent_lds(q.stack_cap);
+    hipLaunchKernelGGL(k_split_persistent, dim3(q.n_groups), dim3(64 * kSplitWaves), lds, s, objs, views, prm,
                        q, cb, fc, balanced);
     return hipGetLastError();
 }
@@ -567,19 +812,23 @@ __global__ __launch_bounds__(256) void k_scatter(const DevObject* __restrict__ o
             r = cb.draw[i];
             g = gpid_of(objs, r);
         }
-        // wave-aggregated bucket cursors: one atomic per (wave, patch)
+        // wave-aggregated bucket cursors: the lanes of one patch elect a
+        // leader, then every leader adds its count in one vector atomic (the
+        // returns arrive together, not one round trip per patch)
         uint64_t m = __ballot(valid);
-        uint32_t slot = 0;
+        int my_leader = 0;
+        uint32_t my_rank = 0, lead_cnt = 0;
         while (m) {
             const int leader = __ffsll((unsigned long long)m) - 1;
             const uint32_t lg = __shfl(g, leader);
             const uint64_t same = __ballot(valid && g == lg) & m;
-            uint32_t b = 0;
-            if (lane == leader) b = cb.offsets[lg] + atomicAdd(&cb.cursor[lg], (uint32_t)__popcll(same));
-            b = __shfl(b, leader);
-            if ((same >> lane) & 1ull) slot = b + lanes_below(same);
+            if ((same >> lane) & 1ull) { my_leader = leader; my_rank = lanes_below(same); }
+            if (lane == leader) lead_cnt = (uint32_t)__popcll(same);
             m &= ~same;
         }
+        uint32_t b = 0;
+        if (lead_cnt) b = cb.offsets[g] + atomicAdd(&cb.cursor[g], lead_cnt);
+        const uint32_t slot = (uint32_t)__shfl((int)b, my_leader) + my_rank;
         if (valid) {
             cb.bucket[slot] = r;
             cb.keys[slot] = canonical_key(r);
@@ -1158,10 +1407,15 @@ hipError_t launch_init_roots(int obj, uint32_t n, RangeRec* out, hipStream_t s)
     return hipGetLastError();
 }
 
+size_t split_persistent_lds(int stack_cap)
+{
+    return ((size_t)stack_cap + kDrawBuf) * sizeof(RangeRec);
+}
+
 int split_persistent_occupancy(size_t lds_bytes)
 {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_split_persistent, 64, lds_bytes) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_split_persistent, 64 * kSplitWaves, lds_bytes) != hipSuccess)
         blocks = 1;
     return blocks < 1 ? 1 : blocks;
 }
@@ -1695,6 +1949,22 @@ __global__ void k_level_to_queues(const RangeRec* __restrict__ list, const uint3
         const uint32_t slot = i / G;
         if ((int64_t)slot < q.stride) q.queue[(int64_t)(i % G) * q.stride + slot] = list[i];
     }
+}
+
+hipError_t launch_split_rebalanced(const DevObject* objs, const DevView* views, const DevParams& prm,
+                                   const SplitQueues& q, const CanonBuffers& cb, FrameCounters* fc,
+                                   hipStream_t s)
+{
+    const uint32_t G = (uint32_t)q.n_groups;
+    hipLaunchKernelGGL(k_level_to_queues, dim3((G + 255) / 256 > 64 ? (G + 255) / 256 : 64), dim3(256), 0, s,
+                       q.left, q.left_cnt, 0, q, q.left_cap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    SplitQueues qt = q;
+    qt.pending = q.left_cnt;
+    qt.step_budget = 0;
+    qt.left_cnt = nullptr;
+    return launch_split_persistent(objs, views, prm, qt, cb, fc, 0, s);
 }
 
 int level_grid(int num_cus, int lanes_per_range, int per_cu)

@@ -97,6 +97,22 @@ def test_every_scheduling_gives_the_same_frame(oracle):
     r.close()
 
 
+def test_split_budgets_give_the_same_frame(oracle):
+    """LOCAL persistent split with step budgets (the left ranges dealt to
+    all groups in a second launch), with and without work sharing."""
+    w = scenes.c2_torus(width=640, height=360)
+    print("case c2_torus 640x360 split budgets", flush=True)
+    ref = oracle.render(w)[0]
+    r, got = render_plain(w)
+    assert_frame_equal(got, ref, "c2-small/default")
+    for sharing, budget in ((0, 0), (0, 1), (0, 3), (0, 8), (1, 3)):
+        r.set_tuning(split_engine=0, work_sharing=sharing, split_budget=budget)
+        r.replay(2, use_graph=True)
+        assert_frame_equal(_frame_of(r), ref, "c2-small/sharing%d-budget%d" % (sharing, budget))
+        assert r.tuning()["split_budget"] == budget
+    r.close()
+
+
 def test_autotune_keeps_the_frame(oracle):
     w = scenes.c2_torus(width=640, height=360)
     print("case c2_torus 640x360 autotune", flush=True)
