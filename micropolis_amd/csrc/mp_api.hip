@@ -97,6 +97,7 @@ struct mp_context {
     uint32_t chunk_cap = 0;
     ShareState* d_share = nullptr;
     unsigned long long* d_keys = nullptr;
+    uint2* d_span = nullptr;
 
     // segment engine (BREADTH / BOUNDED)
     RangeRec* d_seg_a = nullptr; size_t seg_a_cap = 0;
@@ -387,7 +388,7 @@ extern "C" void mp_destroy(mp_context* ctx)
     hipFree(ctx->d_totals); hipHostFree(ctx->h_totals);
     hipFree(ctx->d_queue); hipFree(ctx->d_queue_cnt); hipFree(ctx->d_shared_q); hipFree(ctx->d_shared_cnt);
     hipFree(ctx->d_chunks); hipFree(ctx->d_chunk_count); hipFree(ctx->d_chunk_ready); hipFree(ctx->d_share);
-    hipFree(ctx->d_keys);
+    hipFree(ctx->d_keys); hipFree(ctx->d_span);
     hipFree(ctx->d_visible); hipFree(ctx->d_next);
     hipFree(ctx->d_level_a); hipFree(ctx->d_level_b); hipFree(ctx->d_level_cnt); hipHostFree(ctx->h_level_cnt);
     hipFree(ctx->d_seg_a); hipFree(ctx->d_seg_b); hipFree(ctx->d_flags); hipFree(ctx->d_blk);
@@ -543,7 +544,8 @@ static int ensure_lists(mp_context* ctx, uint32_t cap)
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     int rc;
     if ((rc = dev_alloc(ctx, &ctx->d_draw, cap)) || (rc = dev_alloc(ctx, &ctx->d_bucket, cap)) ||
-        (rc = dev_alloc(ctx, &ctx->d_canon, cap)) || (rc = dev_alloc(ctx, &ctx->d_keys, cap)))
+        (rc = dev_alloc(ctx, &ctx->d_canon, cap)) || (rc = dev_alloc(ctx, &ctx->d_keys, cap)) ||
+        (rc = dev_alloc(ctx, &ctx->d_span, cap)))
         return rc;
     const size_t q = (size_t)cap * ctx->cfg.reyes_patch_size * ctx->cfg.reyes_patch_size;
     if ((rc = dev_alloc(ctx, &ctx->d_quad, q))) return rc;
@@ -563,7 +565,7 @@ static int grow_lists(mp_context* ctx, uint32_t need, uint32_t keep)
     if ((rc = dev_grow(ctx, &ctx->d_draw, &cap, need, keep))) return rc;
     const uint32_t ncap = (uint32_t)cap;
     if ((rc = dev_alloc(ctx, &ctx->d_bucket, ncap)) || (rc = dev_alloc(ctx, &ctx->d_canon, ncap)) ||
-        (rc = dev_alloc(ctx, &ctx->d_keys, ncap)))
+        (rc = dev_alloc(ctx, &ctx->d_keys, ncap)) || (rc = dev_alloc(ctx, &ctx->d_span, ncap)))
         return rc;
     const size_t q = (size_t)ncap * ctx->cfg.reyes_patch_size * ctx->cfg.reyes_patch_size;
     if ((rc = dev_alloc(ctx, &ctx->d_quad, q))) return rc;
@@ -596,6 +598,7 @@ static CanonBuffers canon_buffers(mp_context* ctx)
     cb.bucket = ctx->d_bucket;
     cb.canon = ctx->d_canon;
     cb.keys = ctx->d_keys;
+    cb.span = ctx->d_span;
     cb.cap = ctx->list_cap;
     cb.counts = ctx->d_counts;
     cb.offsets = ctx->d_offsets;
@@ -723,8 +726,9 @@ static int ensure_queues(mp_context* ctx)
             HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_level_cnt, 64 * sizeof(uint32_t), 0));
         }
     }
-    const size_t roots_or_left = std::max<size_t>(ctx->total_roots, ctx->split_engine == 1 || rebalancing(ctx) ? ctx->level_cap : 0);
-    const size_t stride = balanced && ctx->split_engine != 1 ? 1 : (roots_or_left + groups - 1) / groups;
+    // private root queues (ranges left by level launches or a step budget
+    // are read from the level list in place)
+    const size_t stride = balanced && ctx->split_engine != 1 ? 1 : (ctx->total_roots + groups - 1) / groups;
     const size_t entries = std::max<size_t>(1, stride * groups);
     if (entries > ctx->queue_entries) {
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -771,8 +775,7 @@ static SplitQueues split_queues(mp_context* ctx)
     const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
     q.queue = ctx->d_queue;
     q.queue_cnt = ctx->d_queue_cnt;
-    const size_t left = std::max<size_t>(ctx->total_roots, ctx->split_engine == 1 || rebalancing(ctx) ? ctx->level_cap : 0);
-    q.stride = (balanced && ctx->split_engine != 1) ? 0 : (int64_t)((left + ctx->split_groups - 1) / ctx->split_groups);
+    q.stride = (balanced && ctx->split_engine != 1) ? 0 : (int64_t)((ctx->total_roots + ctx->split_groups - 1) / ctx->split_groups);
     q.shared_q = ctx->d_shared_q;
     q.shared_cnt = ctx->d_shared_cnt;
     q.n_groups = ctx->split_groups;
@@ -790,6 +793,7 @@ static SplitQueues split_queues(mp_context* ctx)
     q.left = nullptr;
     q.left_cnt = nullptr;
     q.left_cap = 0;
+    q.strided = nullptr;
     return q;
 }
 
@@ -818,7 +822,8 @@ static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
 {
     const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
-    HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
+    // the first split launch resets the frame counters
+    if (!ctx->total_roots) HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
     SplitQueues q = split_queues(ctx);
     if (ctx->total_roots && ctx->split_engine == 1) {
         if (!ctx->level_grid_size)
@@ -826,21 +831,21 @@ static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
         HIPCHK(ctx, launch_split_levels(ctx->d_objs, (int)ctx->objects.size(), ctx->d_views, ctx->prm,
                                         ctx->total_roots, ctx->d_level_a, ctx->d_level_b, ctx->level_cap,
                                         ctx->d_level_cnt, ctx->level_launches, q, canon_buffers(ctx), ctx->d_fc,
-                                        ctx->level_grid_size, ctx->level_lanes, ctx->stream));
+                                        ctx->level_grid_size, ctx->level_lanes, ctx->d_share, ctx->stream));
     } else if (ctx->total_roots && rebalancing(ctx)) {
         q.step_budget = ctx->split_budget;
         q.left = ctx->d_level_a;
         q.left_cnt = ctx->d_level_cnt + 63;
         q.left_cap = ctx->level_cap;
         HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
-                                       balanced, ctx->stream));
+                                       balanced, ctx->d_fc, ctx->d_share, ctx->stream));
         HIPCHK(ctx, launch_split_persistent(ctx->d_objs, ctx->d_views, ctx->prm, q,
                                             canon_buffers(ctx), ctx->d_fc, balanced, ctx->stream));
         HIPCHK(ctx, launch_split_rebalanced(ctx->d_objs, ctx->d_views, ctx->prm, q, canon_buffers(ctx),
                                             ctx->d_fc, ctx->stream));
     } else if (ctx->total_roots) {
         HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
-                                       balanced, ctx->stream));
+                                       balanced, ctx->d_fc, ctx->d_share, ctx->stream));
         HIPCHK(ctx, launch_split_persistent(ctx->d_objs, ctx->d_views, ctx->prm, q,
                                             canon_buffers(ctx), ctx->d_fc, balanced, ctx->stream));
     }

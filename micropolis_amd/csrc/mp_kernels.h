@@ -38,6 +38,9 @@ struct SplitQueues {
     RangeRec* left;
     uint32_t* left_cnt;
     uint32_t left_cap;
+    // if set: the private queue of group g is strided[g], strided[g + G], ...
+    // (G groups; *pending entries in all, at most left_cap)
+    const RangeRec* strided;
 };
 
 struct CanonBuffers {
@@ -45,6 +48,7 @@ struct CanonBuffers {
     RangeRec* bucket;        // grouped by global patch index
     RangeRec* canon;         // canonical order
     unsigned long long* keys;  // canonical keys of the bucketed records
+    uint2* span;             // [offsets[g], offsets[g + 1]) of each bucketed record's patch
     uint32_t cap;            // capacity of the three lists
     uint32_t* counts;        // per global patch: DRAW count
     uint32_t* offsets;       // exclusive scan of counts (n_gpids + 1)
@@ -56,8 +60,11 @@ hipError_t launch_reset(FrameCounters* fc, ShareState* share, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
+// reset_fc / reset_share (may be null): the frame's counters are reset by the
+// same launch (as launch_reset does)
 hipError_t launch_init_queues(const DevObject* objs, int n_objs, uint32_t total_roots,
-                              const SplitQueues& q, int balanced, hipStream_t s);
+                              const SplitQueues& q, int balanced, FrameCounters* reset_fc,
+                              ShareState* reset_share, hipStream_t s);
 hipError_t launch_split_persistent(const DevObject* objs, const DevView* views,
                                    const DevParams& prm, const SplitQueues& q,
                                    const CanonBuffers& cb, FrameCounters* fc, int balanced,
@@ -100,9 +107,10 @@ int level_grid(int num_cus, int lanes_per_range, int per_cu);
 hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView* views, const DevParams& prm,
                                uint32_t total_roots, RangeRec* list_a, RangeRec* list_b, uint32_t list_cap,
                                uint32_t* level_cnt, int n_levels, const SplitQueues& q,
-                               const CanonBuffers& cb, FrameCounters* fc, int grid, int lanes_per_range, hipStream_t s);
-// After a budgeted persistent split (q.step_budget > 0): deal q.left to the
-// groups' private queues and finish the split without a budget.
+                               const CanonBuffers& cb, FrameCounters* fc, int grid, int lanes_per_range,
+                               ShareState* reset_share, hipStream_t s);
+// After a budgeted persistent split (q.step_budget > 0): finish the split
+// without a budget, group g taking q.left[g], q.left[g + G], ...
 hipError_t launch_split_rebalanced(const DevObject* objs, const DevView* views, const DevParams& prm,
                                    const SplitQueues& q, const CanonBuffers& cb, FrameCounters* fc,
                                    hipStream_t s);

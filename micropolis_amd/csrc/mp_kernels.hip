@@ -26,25 +26,29 @@ namespace mp {
 // ------------------------------------------------------------------------
 // small utilities
 
-__global__ void k_reset(FrameCounters* fc, ShareState* share)
+// Start-of-frame counters (one thread).
+MP_DEV void reset_counters(FrameCounters* fc, ShareState* share)
 {
-    if (threadIdx.x == 0 && share) {
+    if (share) {
         share->head = 0;
         share->tail = 0;
         share->active = 0;
         share->epoch = share->epoch + 1u;
     }
-    if (threadIdx.x == 0) {
-        fc->draw_count = 0;
-        fc->overflow = 0;
-        fc->bounds = 0;
-        fc->slow_blocks = 0;
-        fc->max_stack = 0;
-        fc->total_ranges = 0;
-        fc->atomics = 0;
-        fc->raster_next = 0;
-        fc->uncached_octaves = 0;
-    }
+    fc->draw_count = 0;
+    fc->overflow = 0;
+    fc->bounds = 0;
+    fc->slow_blocks = 0;
+    fc->max_stack = 0;
+    fc->total_ranges = 0;
+    fc->atomics = 0;
+    fc->raster_next = 0;
+    fc->uncached_octaves = 0;
+}
+
+__global__ void k_reset(FrameCounters* fc, ShareState* share)
+{
+    if (threadIdx.x == 0) reset_counters(fc, share);
 }
 
 __global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v)
@@ -110,9 +114,10 @@ MP_DEV uint32_t lanes_below(uint64_t mask)
 // LOCAL / BALANCED: persistent split
 
 __global__ void k_init_queues(const DevObject* __restrict__ objs, int n_objs, uint32_t total,
-                              SplitQueues q, int balanced)
+                              SplitQueues q, int balanced, FrameCounters* reset_fc, ShareState* reset_share)
 {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0 && reset_fc) reset_counters(reset_fc, reset_share);
     if (g < (uint32_t)q.n_groups)
         q.queue_cnt[g] = balanced ? 0 : (int32_t)((total + (uint32_t)(q.n_groups - 1) - g) / (uint32_t)q.n_groups);
     if (balanced && g == 0) *q.shared_cnt = (int32_t)total;
@@ -132,11 +137,12 @@ __global__ void k_init_queues(const DevObject* __restrict__ objs, int n_objs, ui
 }
 
 hipError_t launch_init_queues(const DevObject* objs, int n_objs, uint32_t total_roots,
-                              const SplitQueues& q, int balanced, hipStream_t s)
+                              const SplitQueues& q, int balanced, FrameCounters* reset_fc,
+                              ShareState* reset_share, hipStream_t s)
 {
     uint32_t n = total_roots > (uint32_t)q.n_groups ? total_roots : (uint32_t)q.n_groups;
     hipLaunchKernelGGL(k_init_queues, dim3((n + 255) / 256), dim3(256), 0, s, objs, n_objs,
-                       total_roots, q, balanced);
+                       total_roots, q, balanced, reset_fc, reset_share);
     return hipGetLastError();
 }
 
@@ -351,7 +357,13 @@ __global__ __launch_bounds__(64 * kSplitWaves) void k_split_persistent(const Dev
     bool ok = true;
     RangeRec* myq = q.queue + (int64_t)wid * q.stride;
     int base = 0, sh = 0;
-    int qcnt = q.queue_cnt[wid];
+    int qcnt;
+    if (q.strided) {
+        const uint32_t nl = min(*q.pending, q.left_cap), G = (uint32_t)q.n_groups;
+        qcnt = nl > (uint32_t)wid ? (int)((nl - (uint32_t)wid + G - 1u) / G) : 0;
+    } else {
+        qcnt = q.queue_cnt[wid];
+    }
     uint32_t nbounds = 0, maxsh = 0;
     const uint32_t epoch = q.share ? q.share->epoch : 0u;
     const bool sharing = q.chunk_cap > 0;
@@ -451,7 +463,10 @@ __global__ __launch_bounds__(64 * kSplitWaves) void k_split_persistent(const Dev
         rec.pid = 0; rec.obj_depth = 0; rec.umin = rec.vmin = rec.umax = rec.vmax = 0.0f;
         bool occ = false;
         if (lane < cnt) {
-            rec = (src == 2) ? q.chunks[start + lane] : (src == 1 ? q.shared_q[start + lane] : myq[start + lane]);
+            rec = (src == 2)  ? q.chunks[start + lane]
+                : (src == 1)  ? q.shared_q[start + lane]
+                : q.strided   ? q.strided[(size_t)(start + lane) * (uint32_t)q.n_groups + (uint32_t)wid]
+                              : myq[start + lane];
             occ = true;
         } else if (lane < cnt + scnt) {
             int k = base + sh + lane - cnt;
@@ -832,27 +847,69 @@ __global__ __launch_bounds__(256) void k_scatter(const DevObject* __restrict__ o
         if (valid) {
             cb.bucket[slot] = r;
             cb.keys[slot] = canonical_key(r);
+            cb.span[slot] = make_uint2(cb.offsets[g], cb.offsets[g + 1]);
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_rank(const DevObject* __restrict__ objs, CanonBuffers cb,
-                                              const FrameCounters* __restrict__ fc)
+// Rank of each bucketed range among its patch's ranges (its canonical
+// index = the patch's first slot + the number of its ranges with a smaller
+// key).  A work-group takes 256 consecutive bucketed ranges; the keys of
+// every patch they belong to form one contiguous span, staged in LDS once
+// and counted from there (spans above kRankStage keys count from global).
+constexpr uint32_t kRankStage = 4096;
+
+__global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, const FrameCounters* __restrict__ fc)
 {
+    __shared__ unsigned long long s_key[kRankStage];
+    __shared__ uint32_t s_lo[4], s_hi[4];
     const uint32_t total = fc->total_ranges;
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < total; j += gridDim.x * 256) {
-        const RangeRec r = cb.bucket[j];
-        const uint32_t g = gpid_of(objs, r);
-        const uint32_t lo = cb.offsets[g], hi = cb.offsets[g + 1];
-        const unsigned long long key = cb.keys[j];
-        uint32_t rank = 0;
-        uint32_t k = lo;
-        for (; k + 4 <= hi; k += 4) {
-            const unsigned long long a = cb.keys[k], b = cb.keys[k + 1], c = cb.keys[k + 2], d = cb.keys[k + 3];
-            rank += (a < key) + (b < key) + (c < key) + (d < key);
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t j0 = blockIdx.x * 256u; j0 < total; j0 += gridDim.x * 256u) {
+        const uint32_t j = j0 + tid;
+        const bool valid = j < total;
+        uint2 sp = make_uint2(0xffffffffu, 0u);
+        unsigned long long key = 0ull;
+        RangeRec r;
+        if (valid) {
+            sp = cb.span[j];
+            key = cb.keys[j];
+            r = cb.bucket[j];
         }
-        for (; k < hi; ++k) rank += (cb.keys[k] < key) ? 1u : 0u;
-        cb.canon[lo + rank] = r;
+        uint32_t lo = sp.x, hi = sp.y;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
+            hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
+        }
+        if (lane == 0) { s_lo[wave] = lo; s_hi[wave] = hi; }
+        __syncthreads();
+        const uint32_t L = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
+        const uint32_t H = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
+        const bool staged = H - L <= kRankStage;
+        if (staged)
+            for (uint32_t k = L + tid; k < H; k += 256u) s_key[k - L] = cb.keys[k];
+        __syncthreads();
+        if (valid) {
+            uint32_t rank = 0;
+            if (staged) {
+                // eight LDS reads in flight per step
+                const unsigned long long* b = s_key + (sp.x - L);
+                const uint32_t n = sp.y - sp.x;
+                uint32_t k = 0;
+                for (; k + 8u <= n; k += 8u) {
+                    uint32_t c = 0;
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) c += b[k + t] < key ? 1u : 0u;
+                    rank += c;
+                }
+                for (; k < n; ++k) rank += b[k] < key ? 1u : 0u;
+            } else {
+                for (uint32_t k = sp.x; k < sp.y; ++k) rank += cb.keys[k] < key ? 1u : 0u;
+            }
+            cb.canon[sp.x + rank] = r;
+        }
+        __syncthreads();   // s_key / s_lo reused by the next 256 ranges
     }
 }
 
@@ -867,7 +924,7 @@ hipError_t launch_canonicalize(const DevObject* objs, DevView* views, int n_view
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, objs, cb, fc);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_rank, dim3(grid), dim3(256), 0, s, objs, cb, fc);
+    hipLaunchKernelGGL(k_rank, dim3(num_cus * 4), dim3(256), 0, s, cb, fc);
     return hipGetLastError();
 }
 
@@ -1768,9 +1825,11 @@ hipError_t launch_adjoint(int patch_size, const RangeRec* canon, const DevObject
 namespace mp {
 
 __global__ void k_level_roots(const DevObject* __restrict__ objs, int n_objs, uint32_t total,
-                              RangeRec* out, uint32_t* level_cnt, int n_levels)
+                              RangeRec* out, uint32_t* level_cnt, int n_levels, FrameCounters* reset_fc,
+                              ShareState* reset_share)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0 && reset_fc) reset_counters(reset_fc, reset_share);
     if (g < (uint32_t)n_levels + 1) level_cnt[g] = (g == 0) ? total : 0u;
     if (g >= total) return;
     int lo = 0, hi = n_objs - 1;
@@ -1936,32 +1995,13 @@ __global__ __launch_bounds__(256) void k_level9(const DevObject* __restrict__ ob
     if (lane == 0 && nb) atomicAdd(&fc->bounds, (unsigned long long)nb);
 }
 
-// Leftover ranges after the last launched level go to the persistent
-// engine's private queues (round-robin).
-__global__ void k_level_to_queues(const RangeRec* __restrict__ list, const uint32_t* level_cnt, int L,
-                                  SplitQueues q, uint32_t list_cap)
-{
-    const uint32_t n = min(level_cnt[L], list_cap);
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t G = (uint32_t)q.n_groups;
-    if (g < G) q.queue_cnt[g] = (int32_t)((n + G - 1u - g) / G);
-    for (uint32_t i = g; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t slot = i / G;
-        if ((int64_t)slot < q.stride) q.queue[(int64_t)(i % G) * q.stride + slot] = list[i];
-    }
-}
-
 hipError_t launch_split_rebalanced(const DevObject* objs, const DevView* views, const DevParams& prm,
                                    const SplitQueues& q, const CanonBuffers& cb, FrameCounters* fc,
                                    hipStream_t s)
 {
-    const uint32_t G = (uint32_t)q.n_groups;
-    hipLaunchKernelGGL(k_level_to_queues, dim3((G + 255) / 256 > 64 ? (G + 255) / 256 : 64), dim3(256), 0, s,
-                       q.left, q.left_cnt, 0, q, q.left_cap);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     SplitQueues qt = q;
     qt.pending = q.left_cnt;
+    qt.strided = q.left;
     qt.step_budget = 0;
     qt.left_cnt = nullptr;
     return launch_split_persistent(objs, views, prm, qt, cb, fc, 0, s);
@@ -1984,11 +2024,11 @@ hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView*
                                uint32_t total_roots, RangeRec* list_a, RangeRec* list_b, uint32_t list_cap,
                                uint32_t* level_cnt, int n_levels, const SplitQueues& q,
                                const CanonBuffers& cb, FrameCounters* fc, int grid, int lanes_per_range,
-                               hipStream_t s)
+                               ShareState* reset_share, hipStream_t s)
 {
     const uint32_t n0 = total_roots > (uint32_t)(n_levels + 1) ? total_roots : (uint32_t)(n_levels + 1);
     hipLaunchKernelGGL(k_level_roots, dim3((n0 + 255) / 256), dim3(256), 0, s, objs, n_objs, total_roots,
-                       list_a, level_cnt, n_levels);
+                       list_a, level_cnt, n_levels, fc, reset_share);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     for (int L = 0; L < n_levels; ++L) {
@@ -2003,15 +2043,12 @@ hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView*
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    // anything deeper finishes in the persistent engine
-    const RangeRec* left = (n_levels & 1) ? list_b : list_a;
-    const uint32_t G = (uint32_t)q.n_groups;
-    hipLaunchKernelGGL(k_level_to_queues, dim3((G + 255) / 256 > 64 ? (G + 255) / 256 : 64), dim3(256), 0, s,
-                       left, level_cnt, n_levels, q, list_cap);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    // anything deeper finishes in the persistent engine, group g taking
+    // left[g], left[g + G], ...
     SplitQueues qt = q;
     qt.pending = level_cnt + n_levels;
+    qt.strided = (n_levels & 1) ? list_b : list_a;
+    qt.left_cap = list_cap;
     return launch_split_persistent(objs, views, prm, qt, cb, fc, 0, s);
 }
 
