@@ -186,11 +186,18 @@ def main():
     raster_ms = stages["raster"]
     achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
     traffic = None
+    issue = None
     prof = os.path.join(ROOT, "profiles", "k_raster16_c2_torus_1080p.md")
     if os.path.exists(prof):
         for ln in open(prof):
             if ln.startswith("traffic_bytes_per_launch:"):
                 traffic = int(ln.split(":")[1])
+            if ln.startswith("VALU issue:"):
+                # "VALU issue: I wave instructions against S issue slots"
+                f = ln.replace(",", "").split()
+                issue = {"valu_wave_instructions": int(f[2]), "issue_slots": int(f[6])}
+        if issue:
+            issue["frac"] = round(issue["valu_wave_instructions"] / issue["issue_slots"], 3)
 
     batch = None if args.no_batch else measure_batch(max(3, min(args.steps, 10)), use_graph)
     adjoint = None if args.no_adjoint else run_adjoint_child(max(3, min(args.steps, 10)))
@@ -233,7 +240,10 @@ def main():
         "stages_ms": {k: round(x, 4) for k, x in stages.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "k_raster<16>", "alg_bytes": int(alg_bytes)},
+                     "kernel": "k_raster<16>", "alg_bytes": int(alg_bytes),
+                     # the kernel's actual limit: VALU issue (rocprofv3 counters of
+                     # the same command, one wave64 instruction per CU per cycle)
+                     "valu_issue": issue},
         "cpu_baseline": cpu,
         "batch": batch,
         "adjoint": adjoint,
