@@ -109,6 +109,7 @@ struct mp_context {
     RangeRec* d_draw = nullptr; RangeRec* d_bucket = nullptr; RangeRec* d_canon = nullptr;
     uint32_t list_cap = 0;
     uint32_t* d_counts = nullptr; uint32_t* d_offsets = nullptr; uint32_t* d_cursor = nullptr;
+    uint32_t* d_part = nullptr;
     uint32_t gpid_cap = 0;
     float* d_quad = nullptr; size_t quad_cap = 0;
 
@@ -393,7 +394,7 @@ extern "C" void mp_destroy(mp_context* ctx)
     hipFree(ctx->d_level_a); hipFree(ctx->d_level_b); hipFree(ctx->d_level_cnt); hipHostFree(ctx->h_level_cnt);
     hipFree(ctx->d_seg_a); hipFree(ctx->d_seg_b); hipFree(ctx->d_flags); hipFree(ctx->d_blk);
     hipFree(ctx->d_draw); hipFree(ctx->d_bucket); hipFree(ctx->d_canon);
-    hipFree(ctx->d_counts); hipFree(ctx->d_offsets); hipFree(ctx->d_cursor);
+    hipFree(ctx->d_counts); hipFree(ctx->d_offsets); hipFree(ctx->d_cursor); hipFree(ctx->d_part);
     hipFree(ctx->d_quad);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
@@ -582,7 +583,7 @@ static int ensure_gpids(mp_context* ctx, uint32_t n)
     uint32_t cap = std::max<uint32_t>(n + 1, 1024);
     int rc;
     if ((rc = dev_alloc(ctx, &ctx->d_counts, cap)) || (rc = dev_alloc(ctx, &ctx->d_offsets, cap)) ||
-        (rc = dev_alloc(ctx, &ctx->d_cursor, cap)))
+        (rc = dev_alloc(ctx, &ctx->d_cursor, cap)) || (rc = dev_alloc(ctx, &ctx->d_part, cap / 4096 + 2)))
         return rc;
     HIPCHK(ctx, launch_fill_u32(ctx->d_counts, cap, 0, ctx->stream));
     HIPCHK(ctx, launch_fill_u32(ctx->d_cursor, cap, 0, ctx->stream));
@@ -603,6 +604,7 @@ static CanonBuffers canon_buffers(mp_context* ctx)
     cb.counts = ctx->d_counts;
     cb.offsets = ctx->d_offsets;
     cb.cursor = ctx->d_cursor;
+    cb.part = ctx->d_part;
     cb.n_gpids = ctx->n_gpids;
     return cb;
 }

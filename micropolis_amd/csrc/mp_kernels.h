@@ -53,6 +53,7 @@ struct CanonBuffers {
     uint32_t* counts;        // per global patch: DRAW count
     uint32_t* offsets;       // exclusive scan of counts (n_gpids + 1)
     uint32_t* cursor;        // scatter cursors
+    uint32_t* part;          // block sums of the counts (many patches)
     uint32_t n_gpids;
 };
 

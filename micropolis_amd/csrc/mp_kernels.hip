@@ -803,19 +803,83 @@ __global__ __launch_bounds__(1024) void k_scan_counts(CanonBuffers cb, DevView* 
         cb.offsets[n] = tot;
         fc->total_ranges = tot;
     }
-    __threadfence_block();
-    __syncthreads();
-    for (int v = (int)t; v < n_views; v += 1024) {
-        uint32_t b = cb.offsets[views[v].first_gpid];
-        views[v].range_base = b;
-        views[v].range_count = cb.offsets[views[v].gpid_end] - b;
+}
+
+// Many patches (batches): the same scan over blocks of kScanBlock counts.
+// k_count_sums: one sum per block; k_scan_parts: exclusive scan of the sums
+// (one work-group, running carry); k_scan_apply: each block scans its counts
+// from its block offset, writes the offsets and clears counts and cursors.
+constexpr uint32_t kScanBlock = 4096;   // 1024 threads x 4 counts
+
+__global__ __launch_bounds__(1024) void k_count_sums(CanonBuffers cb)
+{
+    __shared__ uint32_t lw[32];
+    const uint32_t n = cb.n_gpids;
+    const uint32_t i0 = blockIdx.x * kScanBlock + threadIdx.x * 4u;
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+        if (i0 + k < n) s += cb.counts[i0 + k];
+    uint32_t tot;
+    block_exclusive_scan_1024(s, lw, tot);
+    if (threadIdx.x == 0) cb.part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_parts(CanonBuffers cb, uint32_t nb, FrameCounters* fc)
+{
+    __shared__ uint32_t lw[32];
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nb; c0 += 1024u) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t x = i < nb ? cb.part[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan_1024(x, lw, tot);
+        if (i < nb) cb.part[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        cb.offsets[cb.n_gpids] = carry;
+        fc->total_ranges = carry;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_apply(CanonBuffers cb)
+{
+    __shared__ uint32_t lw[32];
+    const uint32_t n = cb.n_gpids;
+    const uint32_t i0 = blockIdx.x * kScanBlock + threadIdx.x * 4u;
+    uint32_t c[4], s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        c[k] = i0 + k < n ? cb.counts[i0 + k] : 0u;
+        s += c[k];
+    }
+    uint32_t tot;
+    uint32_t run = cb.part[blockIdx.x] + block_exclusive_scan_1024(s, lw, tot);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (i0 + k < n) {
+            cb.offsets[i0 + k] = run;
+            run += c[k];
+            cb.counts[i0 + k] = 0;
+            cb.cursor[i0 + k] = 0;
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_scatter(const DevObject* __restrict__ objs, CanonBuffers cb,
-                                                 const FrameCounters* __restrict__ fc)
+                                                 const FrameCounters* __restrict__ fc, DevView* views,
+                                                 int n_views)
 {
     const uint32_t total = fc->total_ranges;
+    if (blockIdx.x == 0) {
+        // per-view range bases (the offsets are final)
+        for (int v = (int)threadIdx.x; v < n_views; v += 256) {
+            const uint32_t b = cb.offsets[views[v].first_gpid];
+            views[v].range_base = b;
+            views[v].range_count = cb.offsets[views[v].gpid_end] - b;
+        }
+    }
     const int lane = threadIdx.x & 63;
     const uint32_t stride = gridDim.x * 256;
     const uint32_t lim = (total + 255u) & ~255u;
@@ -917,11 +981,21 @@ hipError_t launch_canonicalize(const DevObject* objs, DevView* views, int n_view
                                const CanonBuffers& cb, FrameCounters* fc, int num_cus,
                                hipStream_t s)
 {
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, cb, views, n_views, fc);
-    hipError_t e = hipGetLastError();
+    hipError_t e;
+    if (cb.n_gpids <= 2u * kScanBlock) {
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, cb, views, n_views, fc);
+    } else {
+        const uint32_t nb = (cb.n_gpids + kScanBlock - 1u) / kScanBlock;
+        hipLaunchKernelGGL(k_count_sums, dim3(nb), dim3(1024), 0, s, cb);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, s, cb, nb, fc);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, s, cb);
+    }
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int grid = num_cus * 4;
-    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, objs, cb, fc);
+    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, objs, cb, fc, views, n_views);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_rank, dim3(num_cus * 4), dim3(256), 0, s, cb, fc);
