@@ -121,7 +121,8 @@ struct mp_context {
     int raster_grid_size = 0;
     int split_groups = 0;
     int stack_cap = 0;
-    int split_spread = 1;        // persistent split: one sample per lane on narrow steps
+    int split_spread = 2;        // persistent split: 0 three samples per lane; 1 one sample
+                                 // per lane on narrow steps; 2 also two levels on the narrowest
     int split_budget = 8;        // LOCAL persistent split: steps before rebalancing (0: none)
     mp_stats stats;
     std::vector<int32_t> pass_counts;

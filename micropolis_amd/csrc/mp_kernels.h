@@ -30,7 +30,8 @@ struct SplitQueues {
     int32_t linger;          // idle groups wait (bounded) for spilled chunks
     ShareState* share;
     const uint32_t* pending; // if set and zero on entry, there is no work: exit
-    int32_t spread;          // narrow steps: one sample per lane over the group's waves
+    int32_t spread;          // 1: narrow steps one sample per lane; 2: also two levels per
+                             // step on the narrowest (speculative children)
     // step budget: a group that has run step_budget steps (0: no budget)
     // appends its queued and stacked ranges to `left` and exits; a second
     // launch deals them round-robin to all groups

@@ -178,12 +178,79 @@ MP_DEV void wave_sync()
 constexpr int kSplitWaves = 3;
 constexpr int kNarrowStep = 21;
 
-// Sample (iu, iv) of bound()'s samples of rec: eye-space x, y, z, screen x, y.
-MP_DEV void bound_one(const DevObject* __restrict__ objs, const DevView* __restrict__ views,
-                      const RangeRec& rec, int iu, int iv, float (&v)[5])
+// Two levels per step on the narrowest steps (at most kSpecStep ranges): each
+// range gets 21 lanes over the three waves for its 9 samples plus the 12
+// samples its children add in either split direction (a child's other 6
+// samples are the parent's: the ranges are dyadic, k / 2^d with d <= 21 for
+// any range that is evaluated, so u = umin + (umax - umin) * t is exact for
+// t = 0, 1/2, 1 and a child's edge samples repeat the parent's bit for bit).
+// Five lanes per range then take the parent's and the four possible
+// children's decisions at once; the children of the direction the parent
+// chose continue the step as if they had been popped.
+constexpr int kSpecStep = 9;
+constexpr int kSpecSamples = 21;
+
+// Range and grid position of speculative sample s (0..20) of parent p:
+// 0..8 the parent's (iu, iv); 9..11 / 12..14 the middle column of the u-split
+// children; 15..17 / 18..20 the middle row of the v-split children.
+MP_DEV void spec_sample(const RangeRec& p, int s, RangeRec& c, int& iu, int& iv)
 {
-    const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
-    const DevView& P = views[ob.view];
+    c = p;
+    if (s < 9) { iu = s / 3; iv = s - 3 * iu; return; }
+    const float cu = (p.umin + p.umax) * 0.5f, cv = (p.vmin + p.vmax) * 0.5f;   // as split_children
+    c.obj_depth = p.obj_depth + (1u << 24);
+    if (s < 12)      { c.umax = cu; iu = 1; iv = s - 9; }
+    else if (s < 15) { c.umin = cu; iu = 1; iv = s - 12; }
+    else if (s < 18) { c.vmax = cv; iu = s - 15; iv = 1; }
+    else             { c.vmin = cv; iu = s - 18; iv = 1; }
+}
+
+// Speculative sample index of sample (iu, iv) of decision d: 0 the parent,
+// 1 / 2 the u-split children (umax = cu / umin = cu), 3 / 4 the v-split ones.
+MP_DEV int spec_index(int d, int iu, int iv)
+{
+    switch (d) {
+    case 0: return iu * 3 + iv;
+    case 1: return iu == 0 ? iv : (iu == 1 ? 9 + iv : 3 + iv);
+    case 2: return iu == 0 ? 3 + iv : (iu == 1 ? 12 + iv : 6 + iv);
+    case 3: return iv == 0 ? iu * 3 : (iv == 1 ? 15 + iu : iu * 3 + 1);
+    default: return iv == 0 ? iu * 3 + 1 : (iv == 1 ? 18 + iu : iu * 3 + 2);
+    }
+}
+
+MP_DEV void spec_range(const RangeRec& p, int d, RangeRec& c)
+{
+    c = p;
+    if (d == 0) return;
+    const float cu = (p.umin + p.umax) * 0.5f, cv = (p.vmin + p.vmax) * 0.5f;
+    c.obj_depth = p.obj_depth + (1u << 24);
+    if (d == 1) c.umax = cu;
+    else if (d == 2) c.umin = cu;
+    else if (d == 3) c.vmax = cv;
+    else c.vmin = cv;
+}
+
+// Calls f(object, view) for the ranges a wave holds (valid lanes): with a
+// wave-uniform object index when every valid lane has the same object, so
+// the tables are read by scalar loads, else per lane.
+template <typename F>
+MP_DEV void with_object(const DevObject* __restrict__ objs, const DevView* __restrict__ views,
+                        uint32_t obj, bool valid, F&& f)
+{
+    const uint32_t o0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)obj);
+    if (__all(!valid || obj == o0)) {
+        const DevObject& ob = objs[o0];
+        f(ob, views[ob.view]);
+    } else {
+        const DevObject& ob = objs[obj];
+        f(ob, views[ob.view]);
+    }
+}
+
+// Sample (iu, iv) of bound()'s samples of rec: eye-space x, y, z, screen x, y.
+MP_DEV void bound_one(const DevObject& ob, const DevView& P, const RangeRec& rec, int iu, int iv,
+                      float (&v)[5])
+{
     float e[3];
     if (ob.type == MP_TYPE_GREGORY) {
         const float4* cp = ob.cps + (size_t)rec.pid * 20;
@@ -202,11 +269,9 @@ MP_DEV void bound_one(const DevObject* __restrict__ objs, const DevView* __restr
 }
 
 // Column iu (samples iu*3 + 0..2) of bound()'s samples of rec.
-MP_DEV void bound_column(const DevObject* __restrict__ objs, const DevView* __restrict__ views,
-                         const RangeRec& rec, int iu, float (&v)[15])
+MP_DEV void bound_column(const DevObject& ob, const DevView& P, const RangeRec& rec, int iu,
+                         float (&v)[15])
 {
-    const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
-    const DevView& P = views[ob.view];
     float e[3];
     if (ob.type == MP_TYPE_GREGORY) {
         const float4* cp = ob.cps + (size_t)rec.pid * 20;
@@ -328,14 +393,35 @@ __global__ __launch_bounds__(64 * kSplitWaves) void k_split_persistent(const Dev
     if (wave > 0) {
         for (;;) {
             __syncthreads();   // request posted
-            const int cmd = s_cmd;
-            if (cmd < 0) return;
-            if (cmd <= kNarrowStep) {
+            const int req = s_cmd;
+            if (req < 0) return;
+            const int cmd = req & 0xffff, mode = req >> 16;
+            if (mode == 2) {
+                float* s_spec = &s_xch[0][0][0];
+                const int L = wave * 64 + lane, g = L / kSpecSamples, sidx = L - kSpecSamples * g;
+                const bool valid = g < cmd;
+                const RangeRec pr = s_stage[min(g, 63)];
+                RangeRec c;
+                int iu, iv;
+                spec_sample(pr, sidx, c, iu, iv);
+                const bool live = valid && (int)(c.obj_depth >> 24) < prm.max_split_depth - 1;
+                float v[5] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+                with_object(objs, views, pr.obj_depth & 0xffffffu, valid,
+                            [&](const DevObject& ob, const DevView& P) {
+                                if (live) bound_one(ob, P, c, iu, iv, v);
+                            });
+                if (valid)
+#pragma unroll
+                    for (int t = 0; t < 5; ++t) s_spec[(g * kSpecSamples + sidx) * 5 + t] = v[t];
+            } else if (mode == 1) {
                 const int g = lane / 3;
                 const RangeRec rec = s_stage[min(g, 63)];
                 float v[5] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
-                if (g < cmd && (int)(rec.obj_depth >> 24) < prm.max_split_depth - 1)
-                    bound_one(objs, views, rec, wave, lane - 3 * g, v);
+                const bool live = g < cmd && (int)(rec.obj_depth >> 24) < prm.max_split_depth - 1;
+                with_object(objs, views, rec.obj_depth & 0xffffffu, g < cmd,
+                            [&](const DevObject& ob, const DevView& P) {
+                                if (live) bound_one(ob, P, rec, wave, lane - 3 * g, v);
+                            });
 #pragma unroll
                 for (int t = 0; t < 5; ++t) s_xch[wave - 1][t][lane] = v[t];
             } else {
@@ -343,8 +429,11 @@ __global__ __launch_bounds__(64 * kSplitWaves) void k_split_persistent(const Dev
                 float v[15];
 #pragma unroll
                 for (int t = 0; t < 15; ++t) v[t] = 0.0f;
-                if (lane < cmd && (int)(rec.obj_depth >> 24) < prm.max_split_depth - 1)
-                    bound_column(objs, views, rec, wave, v);
+                const bool live = lane < cmd && (int)(rec.obj_depth >> 24) < prm.max_split_depth - 1;
+                with_object(objs, views, rec.obj_depth & 0xffffffu, lane < cmd,
+                            [&](const DevObject& ob, const DevView& P) {
+                                if (live) bound_column(ob, P, rec, wave, v);
+                            });
 #pragma unroll
                 for (int t = 0; t < 15; ++t) s_xch[wave - 1][t][lane] = v[t];
             }
@@ -476,18 +565,80 @@ __global__ __launch_bounds__(64 * kSplitWaves) void k_split_persistent(const Dev
         wave_sync();   // stack reads before this step's pushes
         int flags = 0;
         const int n = cnt + scnt;
-        if (q.spread && n <= kNarrowStep) {
+        if (q.spread >= 2 && n <= kSpecStep) {
+            // narrowest step: the ranges and, speculatively, their children
+            float* s_spec = &s_xch[0][0][0];
+            s_stage[lane] = rec;
+            if (lane == 0) s_cmd = n | (2 << 16);
+            __syncthreads();   // request posted
+            {
+                const int g = lane / kSpecSamples, sidx = lane - kSpecSamples * g;
+                const bool valid = g < n;
+                const RangeRec pr = s_stage[g];
+                RangeRec c;
+                int iu, iv;
+                spec_sample(pr, sidx, c, iu, iv);
+                const bool live = valid && (int)(c.obj_depth >> 24) < prm.max_split_depth - 1;
+                float v[5] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+                with_object(objs, views, pr.obj_depth & 0xffffffu, valid,
+                            [&](const DevObject& ob, const DevView& P) {
+                                if (live) bound_one(ob, P, c, iu, iv, v);
+                            });
+                if (valid)
+#pragma unroll
+                    for (int t = 0; t < 5; ++t) s_spec[(g * kSpecSamples + sidx) * 5 + t] = v[t];
+            }
+            __syncthreads();   // samples written
+            // decision d of range g on lane 5g + d
+            const int g = lane / 5, d = lane - 5 * g;
+            const bool valid = g < n;
+            const RangeRec pr = s_stage[min(g, 63)];
+            RangeRec cr;
+            spec_range(pr, d, cr);
+            const bool live = valid && (int)(cr.obj_depth >> 24) < prm.max_split_depth - 1;
+            float ex[9], ey[9], ez[9], sx[9], sy[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int o = valid ? (g * kSpecSamples + spec_index(d, k / 3, k % 3)) * 5 : 0;
+                ex[k] = s_spec[o]; ey[k] = s_spec[o + 1]; ez[k] = s_spec[o + 2];
+                sx[k] = s_spec[o + 3]; sy[k] = s_spec[o + 4];
+            }
+            int f = 0;
+            with_object(objs, views, pr.obj_depth & 0xffffffu, valid,
+                        [&](const DevObject& ob, const DevView& P) {
+                            if (live)
+                                f = bound_decide(ex, ey, ez, sx, sy, cr.umin, cr.vmin, cr.umax, cr.vmax, P,
+                                                 prm, ob.split_limit);
+                        });
+            // the parent's decision picks which lanes continue the step
+            const int pf = __shfl(f, min(5 * g, 63));
+            const bool psplit = (pf & 2) != 0, udir = (pf & 4) != 0;
+            int out = 0;
+            if (valid) {
+                if (d == 0) out = pf == MP_DRAW ? MP_DRAW : 0;
+                else if (d <= 2) out = (psplit && udir) ? f : 0;
+                else out = (psplit && !udir) ? f : 0;
+            }
+            rec = cr;
+            flags = out;
+            occ = out != 0;
+            // the children of a split parent are bounded in this step too
+            nbounds += 2u * (uint32_t)__popcll(__ballot(valid && d == 0 && psplit));
+        } else if (q.spread && n <= kNarrowStep) {
             // narrow step (the deep tail of a split tree): nine lanes per
             // range over the three waves, one sample each
             s_stage[lane] = rec;
-            if (lane == 0) s_cmd = n;
+            if (lane == 0) s_cmd = n | (1 << 16);
             __syncthreads();   // request posted
             const int g = lane / 3, iv = lane - 3 * g;
             const RangeRec rg = s_stage[min(g, 63)];
             const bool valid = g < n;
             const bool live = valid && (int)(rg.obj_depth >> 24) < prm.max_split_depth - 1;
             float v[5] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
-            if (live) bound_one(objs, views, rg, 0, iv, v);
+            with_object(objs, views, rg.obj_depth & 0xffffffu, valid,
+                        [&](const DevObject& ob, const DevView& P) {
+                            if (live) bound_one(ob, P, rg, 0, iv, v);
+                        });
             __syncthreads();   // samples written
             // sample (iu, iv) of range g: wave iu, lane 3g + iv
             float ex[9], ey[9], ez[9], sx[9], sy[9];
@@ -506,11 +657,12 @@ __global__ __launch_bounds__(64 * kSplitWaves) void k_split_persistent(const Dev
             }
             rec = rg;
             occ = valid && iv == 0;
-            if (occ && live) {
-                const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
-                flags = bound_decide(ex, ey, ez, sx, sy, rec.umin, rec.vmin, rec.umax, rec.vmax,
-                                     views[ob.view], prm, ob.split_limit);
-            }
+            with_object(objs, views, rec.obj_depth & 0xffffffu, occ,
+                        [&](const DevObject& ob, const DevView& P) {
+                            if (occ && live)
+                                flags = bound_decide(ex, ey, ez, sx, sy, rec.umin, rec.vmin, rec.umax,
+                                                     rec.vmax, P, prm, ob.split_limit);
+                        });
         } else {
             // one sample column per wave
             s_stage[lane] = rec;
@@ -520,30 +672,34 @@ __global__ __launch_bounds__(64 * kSplitWaves) void k_split_persistent(const Dev
             float c0[15];
 #pragma unroll
             for (int t = 0; t < 15; ++t) c0[t] = 0.0f;
-            if (live) bound_column(objs, views, rec, 0, c0);
+            with_object(objs, views, rec.obj_depth & 0xffffffu, occ,
+                        [&](const DevObject& ob, const DevView& P) {
+                            if (live) bound_column(ob, P, rec, 0, c0);
+                        });
             __syncthreads();   // samples written
-            if (live) {
-                float ex[9], ey[9], ez[9], sx[9], sy[9];
+            float ex[9], ey[9], ez[9], sx[9], sy[9];
+#pragma unroll
+            for (int iv = 0; iv < 3; ++iv) {
+                ex[iv] = c0[iv]; ey[iv] = c0[3 + iv]; ez[iv] = c0[6 + iv];
+                sx[iv] = c0[9 + iv]; sy[iv] = c0[12 + iv];
+            }
+#pragma unroll
+            for (int w = 1; w < kSplitWaves; ++w)
 #pragma unroll
                 for (int iv = 0; iv < 3; ++iv) {
-                    ex[iv] = c0[iv]; ey[iv] = c0[3 + iv]; ez[iv] = c0[6 + iv];
-                    sx[iv] = c0[9 + iv]; sy[iv] = c0[12 + iv];
+                    const int k = w * 3 + iv;
+                    ex[k] = s_xch[w - 1][iv][lane];
+                    ey[k] = s_xch[w - 1][3 + iv][lane];
+                    ez[k] = s_xch[w - 1][6 + iv][lane];
+                    sx[k] = s_xch[w - 1][9 + iv][lane];
+                    sy[k] = s_xch[w - 1][12 + iv][lane];
                 }
-#pragma unroll
-                for (int w = 1; w < kSplitWaves; ++w)
-#pragma unroll
-                    for (int iv = 0; iv < 3; ++iv) {
-                        const int k = w * 3 + iv;
-                        ex[k] = s_xch[w - 1][iv][lane];
-                        ey[k] = s_xch[w - 1][3 + iv][lane];
-                        ez[k] = s_xch[w - 1][6 + iv][lane];
-                        sx[k] = s_xch[w - 1][9 + iv][lane];
-                        sy[k] = s_xch[w - 1][12 + iv][lane];
-                    }
-                const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
-                flags = bound_decide(ex, ey, ez, sx, sy, rec.umin, rec.vmin, rec.umax, rec.vmax,
-                                     views[ob.view], prm, ob.split_limit);
-            }
+            with_object(objs, views, rec.obj_depth & 0xffffffu, occ,
+                        [&](const DevObject& ob, const DevView& P) {
+                            if (live)
+                                flags = bound_decide(ex, ey, ez, sx, sy, rec.umin, rec.vmin, rec.umax,
+                                                     rec.vmax, P, prm, ob.split_limit);
+                        });
         }
         nbounds += (uint32_t)n;
 
