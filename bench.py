@@ -58,7 +58,7 @@ def measure_batch(steps, use_graph):
     return {"workload": w.name, "scenes": len(w.views), "patches": w.patch_count,
             "micropolygons": int(st["micropolygons"]), "plain_64_frames_ms": round(plain_ms, 4),
             "batched_ms": round(batch_ms, 4), "speedup": round(plain_ms / batch_ms, 3),
-            "batched_tuning": {k: tuning[k] for k in ("split_engine", "work_sharing", "split_levels", "split_budget", "raster_per_cu", "raster_chunk")},
+            "batched_tuning": {k: tuning[k] for k in ("split_engine", "work_sharing", "split_levels", "split_budget", "split_waves", "raster_per_cu", "raster_chunk")},
             "same_micropolygons": int(mpolys) == int(st["micropolygons"])}
 
 
@@ -234,7 +234,7 @@ def main():
         "data": "synthetic (seeded torus patches, value-noise displacement)",
         "config": {"workload": w.name, "patches": P, "width": 1920, "height": 1080,
                    "reyes_patch_size": N, "bound_n_split_limit": 16, "method": "LOCAL",
-                   "graph": use_graph, "tuning": {k: tuning[k] for k in ("split_engine", "work_sharing", "split_levels", "split_budget", "raster_per_cu", "raster_chunk")},
+                   "graph": use_graph, "tuning": {k: tuning[k] for k in ("split_engine", "work_sharing", "split_levels", "split_budget", "split_waves", "raster_per_cu", "raster_chunk")},
                    "ranges": st["diced_ranges"], "micropolygons": mpolys,
                    "coverage_samples": st["coverage_samples"]},
         "stages_ms": {k: round(x, 4) for k, x in stages.items()},

@@ -239,6 +239,10 @@ typedef struct mp_tuning {
                                 a persistent group runs before its queued and
                                 stacked ranges are dealt to all groups for a
                                 second launch (0 = one launch) */
+    int32_t split_waves;     /* persistent split kernel build: 2 = all registers,
+                                fewest groups (latency-bound frames); 3 = capped
+                                registers, one more group per CU (frames with
+                                many ranges); 0 = 2 */
     double  frame_ms;        /* mp_autotune: replay time of the chosen setting */
 } mp_tuning;
 int mp_get_tuning(mp_context* ctx, mp_tuning* tuning);

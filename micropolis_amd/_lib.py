@@ -63,7 +63,7 @@ class mp_stats(C.Structure):
 class mp_tuning(C.Structure):
     _fields_ = [("split_engine", C.c_int32), ("work_sharing", C.c_int32),
                 ("raster_per_cu", C.c_int32), ("raster_chunk", C.c_int32),
-                ("split_levels", C.c_int32), ("split_budget", C.c_int32),
+                ("split_levels", C.c_int32), ("split_budget", C.c_int32), ("split_waves", C.c_int32),
                 ("frame_ms", C.c_double)]
 
 

@@ -120,6 +120,7 @@ struct mp_context {
     int max_pixels = 0;
     int raster_grid_size = 0;
     int split_groups = 0;
+    int split_waves = 2;         // persistent split kernel build: 2 or 3 waves per SIMD
     int stack_cap = 0;
     int split_spread = 2;        // persistent split: 0 three samples per lane; 1 one sample
                                  // per lane on narrow steps; 2 also two levels on the narrowest
@@ -360,6 +361,7 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     if (const char* e = getenv("MP_STACK_CAP")) ctx->stack_cap = std::max(256, atoi(e));
     if (const char* e = getenv("MP_SPLIT_SPREAD")) ctx->split_spread = atoi(e);
     if (const char* e = getenv("MP_SPLIT_BUDGET")) ctx->split_budget = std::max(0, atoi(e));
+    if (const char* e = getenv("MP_SPLIT_WAVES")) ctx->split_waves = atoi(e) == 3 ? 3 : 2;
     if (const char* e = getenv("MP_SPLIT_ENGINE")) ctx->split_engine = atoi(e);
     if (const char* e = getenv("MP_CPU_THREADS")) ctx->cpu_threads = atoi(e);
     if (const char* e = getenv("MP_LEVEL_LANES")) ctx->level_lanes = atoi(e) == 1 ? 1 : 9;
@@ -712,7 +714,7 @@ static int ensure_queues(mp_context* ctx)
     const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
     int groups = ctx->cfg.local_bns_work_groups;
     if (groups <= 0) {
-        groups = ctx->num_cus * split_persistent_occupancy(split_persistent_lds(ctx->stack_cap));
+        groups = ctx->num_cus * split_persistent_occupancy(split_persistent_lds(ctx->stack_cap), ctx->split_waves);
     }
     ctx->split_groups = groups;
     if (ctx->split_engine == 1 || rebalancing(ctx)) {
@@ -797,6 +799,7 @@ static SplitQueues split_queues(mp_context* ctx)
     q.left_cnt = nullptr;
     q.left_cap = 0;
     q.strided = nullptr;
+    q.waves = ctx->split_waves;
     return q;
 }
 
@@ -1414,6 +1417,7 @@ extern "C" int mp_get_tuning(mp_context* ctx, mp_tuning* t)
     t->raster_chunk = ctx->prm.raster_chunk;
     t->split_levels = ctx->level_limit;
     t->split_budget = ctx->split_budget;
+    t->split_waves = ctx->split_waves;
     t->frame_ms = 0.0;
     return MP_OK;
 }
@@ -1428,6 +1432,7 @@ static int apply_tuning(mp_context* ctx, const mp_tuning& t)
     ctx->prm.raster_chunk = t.raster_chunk;
     ctx->level_limit = std::max(0, t.split_levels);
     ctx->split_budget = std::max(0, t.split_budget);
+    ctx->split_waves = t.split_waves == 3 ? 3 : 2;
     ctx->level_grid_size = 0;
     if (ctx->split_engine == 1)
         ctx->level_launches = ctx->level_limit > 0 ? std::min(ctx->level_limit, ctx->cfg.max_split_depth)
@@ -1449,7 +1454,8 @@ extern "C" int mp_set_tuning(mp_context* ctx, const mp_tuning* t)
     if (!ctx || !t) return MP_E_INVALID;
     if (t->split_engine < 0 || t->split_engine > 1 || t->raster_per_cu < 0 || t->raster_per_cu > 64 ||
         t->raster_chunk < -1 || t->raster_chunk > 4096 || t->split_levels < 0 || t->split_levels > 23 ||
-        t->split_budget < 0 || t->split_budget > 4096)
+        t->split_budget < 0 || t->split_budget > 4096 ||
+        (t->split_waves != 0 && t->split_waves != 2 && t->split_waves != 3))
         return fail(ctx, MP_E_INVALID, "tuning out of range");
     mp_tuning u = *t;
     if (u.raster_per_cu == 0) u.raster_per_cu = raster_grid(ctx->cfg.reyes_patch_size, ctx->num_cus, ctx->prm) / ctx->num_cus;
@@ -1502,12 +1508,16 @@ extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
     for (int ws : { 0, 1 }) {           // persistent split
         for (int budget : { 0, 5, 8 }) {    // (a budget applies to LOCAL frames without sharing)
             if (budget && (ws || ctx->cfg.bound_n_split_method == MP_BNS_BALANCED)) continue;
-            mp_tuning c = cur;
-            c.split_engine = 0;
-            c.work_sharing = ws;
-            c.split_levels = 0;
-            c.split_budget = budget;
-            cand.push_back(c);
+            for (int waves : { 2, 3 }) {    // fat groups / more groups
+                if (waves == 3 && (ws || budget == 5)) continue;
+                mp_tuning c = cur;
+                c.split_engine = 0;
+                c.work_sharing = ws;
+                c.split_levels = 0;
+                c.split_budget = budget;
+                c.split_waves = waves;
+                cand.push_back(c);
+            }
         }
     }
     for (int lv : { 0, 3, 5 }) {         // level launches, then the persistent tail
@@ -1516,6 +1526,7 @@ extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
         c.work_sharing = 0;
         c.split_levels = lv;
         c.split_budget = 0;
+        c.split_waves = 2;
         cand.push_back(c);
     }
     b = pick_fastest(ctx, cand, replays);

@@ -42,6 +42,7 @@ struct SplitQueues {
     // if set: the private queue of group g is strided[g], strided[g + G], ...
     // (G groups; *pending entries in all, at most left_cap)
     const RangeRec* strided;
+    int32_t waves;           // kernel build: 2 or 3 waves per SIMD (registers vs groups)
 };
 
 struct CanonBuffers {
@@ -117,7 +118,7 @@ hipError_t launch_split_rebalanced(const DevObject* objs, const DevView* views, 
                                    const SplitQueues& q, const CanonBuffers& cb, FrameCounters* fc,
                                    hipStream_t s);
 hipError_t launch_xyz_to_cps(const float* xyz, float4* cps, size_t n, hipStream_t s);
-int split_persistent_occupancy(size_t lds_bytes);
+int split_persistent_occupancy(size_t lds_bytes, int waves);   // waves: 2 or 3 (SplitQueues::waves)
 size_t split_persistent_lds(int stack_cap);   // LDS bytes of one persistent split group
 
 }  // namespace mp

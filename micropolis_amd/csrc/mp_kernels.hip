@@ -376,7 +376,11 @@ MP_DEV int try_pop_chunk(const SplitQueues& q, uint32_t epoch, FrameCounters* fc
 // because its publisher drains the chunk queue before it exits.  The sample
 // waves wait at the work-group barrier between the controller's requests;
 // a negative request count releases them.
-__global__ __launch_bounds__(64 * kSplitWaves) void k_split_persistent(const DevObject* __restrict__ objs,
+// WPE: waves per SIMD the kernel is built for.  2 keeps every register (the
+// fewest, fattest groups: latency-bound frames); 3 caps the registers (some
+// spills) for one more group per CU (frames with plenty of ranges).
+template <int WPE>
+__global__ __launch_bounds__(64 * kSplitWaves) __attribute__((amdgpu_waves_per_eu(WPE))) void k_split_persistent(const DevObject* __restrict__ objs,
                                                                        const DevView* __restrict__ views,
                                                                        DevParams prm, SplitQueues q,
                                                                        CanonBuffers cb, FrameCounters* fc,
@@ -775,7 +779,11 @@ hipError_t launch_split_persistent(const DevObject* objs, const DevView* views,
                                    hipStream_t s)
 {
     size_t lds = split_persistent_lds(q.stack_cap);
-    hipLaunchKernelGGL(k_split_persistent, dim3(q.n_groups), dim3(64 * kSplitWaves), lds, s, objs, views, prm,
+    if (q.waves == 3)
+        hipLaunchKernelGGL(k_split_persistent<3>, dim3(q.n_groups), dim3(64 * kSplitWaves), lds, s, objs, views,
+                           prm, q, cb, fc, balanced);
+    else
+        hipLaunchKernelGGL(k_split_persistent<2>, dim3(q.n_groups), dim3(64 * kSplitWaves), lds, s, objs, views, prm,
                        q, cb, fc, balanced);
     return hipGetLastError();
 }
@@ -1699,10 +1707,13 @@ size_t split_persistent_lds(int stack_cap)
     return ((size_t)stack_cap + kDrawBuf) * sizeof(RangeRec);
 }
 
-int split_persistent_occupancy(size_t lds_bytes)
+int split_persistent_occupancy(size_t lds_bytes, int waves)
 {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_split_persistent, 64 * kSplitWaves, lds_bytes) != hipSuccess)
+    const hipError_t e = waves == 3
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_split_persistent<3>, 64 * kSplitWaves, lds_bytes)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_split_persistent<2>, 64 * kSplitWaves, lds_bytes);
+    if (e != hipSuccess)
         blocks = 1;
     return blocks < 1 ? 1 : blocks;
 }

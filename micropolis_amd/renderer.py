@@ -240,12 +240,13 @@ class Renderer:
         self._check(self._lib.mp_get_tuning(self._ctx, C.byref(t)))
         return {"split_engine": t.split_engine, "work_sharing": t.work_sharing,
                 "raster_per_cu": t.raster_per_cu, "raster_chunk": t.raster_chunk,
-                "split_levels": t.split_levels, "split_budget": t.split_budget}
+                "split_levels": t.split_levels, "split_budget": t.split_budget,
+                "split_waves": t.split_waves}
 
     def set_tuning(self, split_engine=0, work_sharing=0, raster_per_cu=0, raster_chunk=0, split_levels=0,
-                   split_budget=0):
+                   split_budget=0, split_waves=2):
         t = _lib.mp_tuning(int(split_engine), int(work_sharing), int(raster_per_cu), int(raster_chunk),
-                           int(split_levels), int(split_budget), 0.0)
+                           int(split_levels), int(split_budget), int(split_waves), 0.0)
         self._check(self._lib.mp_set_tuning(self._ctx, C.byref(t)))
 
     def autotune(self, replays=5):
@@ -255,7 +256,8 @@ class Renderer:
         self._check(self._lib.mp_autotune(self._ctx, int(replays), C.byref(t)))
         return {"split_engine": t.split_engine, "work_sharing": t.work_sharing,
                 "raster_per_cu": t.raster_per_cu, "raster_chunk": t.raster_chunk,
-                "split_levels": t.split_levels, "split_budget": t.split_budget, "frame_ms": t.frame_ms}
+                "split_levels": t.split_levels, "split_budget": t.split_budget,
+                "split_waves": t.split_waves, "frame_ms": t.frame_ms}
 
     def replay(self, n, use_graph=False):
         ms = C.c_double()

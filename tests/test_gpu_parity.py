@@ -99,17 +99,19 @@ def test_every_scheduling_gives_the_same_frame(oracle):
 
 def test_split_budgets_give_the_same_frame(oracle):
     """LOCAL persistent split with step budgets (the left ranges dealt to
-    all groups in a second launch), with and without work sharing."""
+    all groups in a second launch), with and without work sharing, in both
+    register builds of the split kernel."""
     w = scenes.c2_torus(width=640, height=360)
     print("case c2_torus 640x360 split budgets", flush=True)
     ref = oracle.render(w)[0]
     r, got = render_plain(w)
     assert_frame_equal(got, ref, "c2-small/default")
-    for sharing, budget in ((0, 0), (0, 1), (0, 3), (0, 8), (1, 3)):
-        r.set_tuning(split_engine=0, work_sharing=sharing, split_budget=budget)
+    for sharing, budget, waves in ((0, 0, 2), (0, 1, 2), (0, 3, 2), (0, 8, 2), (1, 3, 2), (0, 0, 3), (0, 8, 3)):
+        r.set_tuning(split_engine=0, work_sharing=sharing, split_budget=budget, split_waves=waves)
         r.replay(2, use_graph=True)
-        assert_frame_equal(_frame_of(r), ref, "c2-small/sharing%d-budget%d" % (sharing, budget))
-        assert r.tuning()["split_budget"] == budget
+        assert_frame_equal(_frame_of(r), ref, "c2-small/sharing%d-budget%d-waves%d" % (sharing, budget, waves))
+        t = r.tuning()
+        assert t["split_budget"] == budget and t["split_waves"] == waves
     r.close()
 
 
