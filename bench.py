@@ -185,6 +185,20 @@ def main():
     alg_bytes = st["diced_ranges"] * (24 + 4 * N * N) + P * 16 * 16 + st["coverage_samples"] * 8
     raster_ms = stages["raster"]
     achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
+
+    # the whole frame against the HBM + 64-bit atomic roofline: compulsory
+    # bytes = control points once (16 x 12 B) + per pixel of the padded
+    # frame clear 8 + resolve read 8 + RGBA8 4 + depth 4 + range records in
+    # and out of the split (2 x 20 B per bound); atomics = coverage samples,
+    # priced at the measured rate of 64-bit atomic minimum to distinct keys
+    W, H = 1920, 1080
+    H_pad = (H + 7) // 8 * 8
+    frame_bytes = P * 16 * 12 + W * H_pad * (8 + 8 + 4 + 4) + 2 * 20 * st["bounded_patches"]
+    atomic_rate = r.measure_atomic_rate(1 << 25)
+    t_roof = max(frame_bytes / (HBM_PEAK_GBS * 1e9), st["coverage_samples"] / atomic_rate)
+    frame_roofline = {"alg_bytes": int(frame_bytes), "atomics": int(st["coverage_samples"]),
+                      "atomic_rate_per_s": round(atomic_rate, 1), "t_roof_us": round(t_roof * 1e6, 2),
+                      "t_frame_us": round(ms_wall * 1e3, 2), "frac": round(t_roof / (ms_wall * 1e-3), 5)}
     traffic = None
     issue = None
     prof = os.path.join(ROOT, "profiles", "k_raster16_c2_torus_1080p.md")
@@ -244,6 +258,7 @@ def main():
                      # the kernel's actual limit: VALU issue (rocprofv3 counters of
                      # the same command, one wave64 instruction per CU per cycle)
                      "valu_issue": issue},
+        "frame_roofline": frame_roofline,
         "cpu_baseline": cpu,
         "batch": batch,
         "adjoint": adjoint,

@@ -261,6 +261,13 @@ int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen);
  * of split, canonical ordering, raster (dice+shade+sample) and resolve. */
 int mp_replay_stages(mp_context* ctx, int32_t n, double stage_ms[4]);
 
+/* Measurement (no reference counterpart): the device's rate of 64-bit
+ * atomic minimum updates to distinct addresses, each wave covering 64
+ * consecutive keys (the raster's key update, relaxed, device scope, no
+ * return), on `n` keys (best of three passes).  Feeds the frame's
+ * HBM + atomic roofline; allocates and frees its own buffer. */
+int mp_measure_atomic_rate(mp_context* ctx, uint32_t n, double* atomics_per_second);
+
 /* ---- differentiable extension (no reference counterpart) ----
  * Reverse pass of dice + displacement + flat shade + z-resolve for the last
  * finished frame or batch, with visibility held fixed to its visible-id

@@ -1538,6 +1538,32 @@ extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
     return MP_OK;
 }
 
+extern "C" int mp_measure_atomic_rate(mp_context* ctx, uint32_t n, double* atomics_per_second)
+{
+    if (!ctx || !atomics_per_second || n == 0 || n > (1u << 30)) return MP_E_INVALID;
+    unsigned long long* keys = nullptr;
+    if (hipMalloc((void**)&keys, (size_t)n * sizeof(unsigned long long)) != hipSuccess)
+        return fail(ctx, MP_E_NOMEM, "atomic-rate buffer");
+    double best = 0.0;
+    int rc = MP_OK;
+    for (int pass = 0; pass < 4 && rc == MP_OK; ++pass) {
+        float ms = 0.0f;
+        if (launch_fill_u64(keys, n, MP_EMPTY_KEY, ctx->stream) != hipSuccess ||
+            hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess ||
+            launch_atomic_rate(keys, n, ctx->stream) != hipSuccess ||
+            hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess ||
+            hipEventSynchronize(ctx->ev1) != hipSuccess ||
+            hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) {
+            rc = fail(ctx, MP_E_HIP, "atomic-rate kernel");
+            break;
+        }
+        if (pass > 0 && ms > 0.0f) best = std::max(best, (double)n / ((double)ms * 1e-3));   // pass 0 warms up
+    }
+    hipFree(keys);
+    if (rc == MP_OK) *atomics_per_second = best;
+    return rc;
+}
+
 extern "C" int mp_replay_stages(mp_context* ctx, int32_t n, double stage_ms[4])
 {
     if (!ctx || n < 1 || !stage_ms) return MP_E_INVALID;

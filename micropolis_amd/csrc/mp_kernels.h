@@ -61,6 +61,7 @@ struct CanonBuffers {
 
 hipError_t launch_reset(FrameCounters* fc, ShareState* share, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_atomic_rate(unsigned long long* keys, uint32_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // reset_fc / reset_share (may be null): the frame's counters are reset by the

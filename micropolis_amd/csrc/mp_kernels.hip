@@ -79,6 +79,19 @@ static unsigned fill_grid(size_t n)
     return (unsigned)b;
 }
 
+// 64-bit key_min over n distinct keys, one per thread (a measurement kernel)
+__global__ __launch_bounds__(256) void k_atomic_rate(unsigned long long* keys, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) key_min(keys + i, ((unsigned long long)(i * 2654435761u) << 32) | i);
+}
+
+hipError_t launch_atomic_rate(unsigned long long* keys, uint32_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_atomic_rate, dim3((n + 255u) / 256u), dim3(256), 0, s, keys, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fill_u64, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);

@@ -221,6 +221,12 @@ class Renderer:
     def update_patches_device(self, handle, xyz_ptr, n_points):
         self._check(self._lib.mp_update_patches_device(self._ctx, int(handle), xyz_ptr, int(n_points)))
 
+    def measure_atomic_rate(self, n=1 << 25):
+        """64-bit atomic-min updates per second to distinct keys (device)."""
+        r = C.c_double()
+        self._check(self._lib.mp_measure_atomic_rate(self._ctx, int(n), C.byref(r)))
+        return r.value
+
     def replay_stages(self, n):
         ms = (C.c_double * 4)()
         self._check(self._lib.mp_replay_stages(self._ctx, int(n), ms))

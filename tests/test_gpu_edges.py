@@ -124,3 +124,12 @@ def test_split_depth_limit(oracle, depth):
     r, got, ref = _render_both(oracle, w, "depth limit %d" % depth, max_split_depth=depth, displacement=0)
     assert (ref["ranges"]["depth"] < depth).all() if len(ref["ranges"]) else True
     r.close()
+
+
+def test_atomic_rate_measurement():
+    """The measurement kernel of the frame roofline runs and reports a rate."""
+    w = scenes.c2_torus(width=64, height=64)
+    r, got = render_plain(w, displacement=0)
+    rate = r.measure_atomic_rate(1 << 20)
+    assert 1e8 < rate < 1e14
+    r.close()
