@@ -344,6 +344,8 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     if (const char* e = getenv("MP_NOISE_CACHE")) ctx->prm.noise_cache = atoi(e);
     ctx->prm.noise_wave_cells = 2;   // lane window (0 per lane, 1 wave-uniform cell)
     ctx->prm.raster_chunk = 0;   // static range stride (see mp_tuning)
+    ctx->prm.raster_one_wave = 1;   // N = 16: one wave per work-group (no waits between waves)
+    if (const char* e = getenv("MP_RASTER_ONE_WAVE")) ctx->prm.raster_one_wave = atoi(e) != 0;
     if (const char* e = getenv("MP_RASTER_CHUNK")) ctx->prm.raster_chunk = atoi(e);
     if (const char* e = getenv("MP_NOISE_WAVE_CELLS")) ctx->prm.noise_wave_cells = atoi(e);
     ctx->raster_grid_size = raster_grid(cfg->reyes_patch_size, ctx->num_cus, ctx->prm);

@@ -17,6 +17,7 @@
 //                                  kernels/reyes.cl); 64-bit depth|id atomicMin
 //  resolve    k_resolve            key buffer -> RGBA8 / depth / ids, and clear
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "mp_device.h"
 #include "mp_kernels.h"
@@ -1241,8 +1242,126 @@ constexpr int kCacheEntries = 512;
 
 // MODE 0: no displacement; 1: displacement, lattice values hashed per use;
 // 2: displacement with the per-range LDS lattice cache.
-template <int N, int MODE>
-__global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ canon,
+// Shade (reyes.cl:84-199) and sample (reyes.cl:270-398) one 8x8 block of a
+// diced range: the calling wave's lane (u, v) owns quad (u, v); s_pos /
+// s_pxl / s_dep hold the range's (N+1)^2 grid; r is its canonical index.
+// Eye-space grid position as the shading reads it: x, y, z (w unused).
+struct EyePos {
+    float x, y, z;
+};
+MP_DEV float4 eye4(const EyePos& p) { return make_float4(p.x, p.y, p.z, 0.0f); }
+MP_DEV float4 eye4(const float4& p) { return p; }
+MP_DEV void set_pos(EyePos& d, float x, float y, float z, float) { d.x = x; d.y = y; d.z = z; }
+MP_DEV void set_pos(float4& d, float x, float y, float z, float w) { d = make_float4(x, y, z, w); }
+
+template <int N, typename PT>
+MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep, int u, int v, int lane,
+                         const DevView& vw, const DevParams& prm, uint32_t r, float* __restrict__ quad_store,
+                         uint32_t& slow, uint32_t& natom)
+{
+    constexpr int G1 = N + 1;
+    constexpr int Q = N * N;
+    const int g0 = u + v * G1;
+    const int sh = prm.subpixel_bits;
+    // ---- shade (reyes.cl:84-199); one wave = one 8x8 block ----
+    const float4 p0 = eye4(s_pos[g0]), p1 = eye4(s_pos[g0 + 1]), p2 = eye4(s_pos[g0 + G1]),
+                 p3 = eye4(s_pos[g0 + G1 + 1]);
+    const int2 x0 = s_pxl[g0], x1 = s_pxl[g0 + 1], x2 = s_pxl[g0 + G1], x3 = s_pxl[g0 + G1 + 1];
+    const float d0 = s_dep[g0], d1 = s_dep[g0 + 1], d2 = s_dep[g0 + G1], d3 = s_dep[g0 + G1 + 1];
+    const bool zero = (p0.z == 0.0f) || (p1.z == 0.0f) || (p2.z == 0.0f) || (p3.z == 0.0f);
+    const bool allnormal = !__any(zero);
+    const int32_t vmaxx = (vw.width << sh) - 1, vmaxy = (vw.height << sh) - 1;
+    bool ff = true;
+    if (prm.backface_culling) {
+        const int32_t d1x = wsub(x1.x, x0.x), d1y = wsub(x1.y, x0.y);
+        const int32_t d2x = wsub(x2.x, x0.x), d2y = wsub(x2.y, x0.y);
+        const int32_t d3x = wsub(x3.x, x0.x), d3y = wsub(x3.y, x0.y);
+        ff = (wsub(wmul(d1x, d3y), wmul(d3x, d1y)) < 0) || (wsub(wmul(d3x, d2y), wmul(d2x, d3y)) < 0);
+    }
+    const int32_t qminx = min(min(x0.x, x1.x), min(x2.x, x3.x));
+    const int32_t qminy = min(min(x0.y, x1.y), min(x2.y, x3.y));
+    const int32_t qmaxx = max(max(x0.x, x1.x), max(x2.x, x3.x));
+    const int32_t qmaxy = max(max(x0.y, x1.y), max(x2.y, x3.y));
+    int32_t xmin = wave_min_i(ff ? min(vmaxx, qminx) : vmaxx);
+    int32_t ymin = wave_min_i(ff ? min(vmaxy, qminy) : vmaxy);
+    int32_t xmax = wave_max_i(ff ? max(0, qmaxx) : 0);
+    int32_t ymax = wave_max_i(ff ? max(0, qmaxy) : 0);
+    xmin = max(0, xmin); ymin = max(0, ymin);
+    xmax = min(vmaxx, xmax); ymax = min(vmaxy, ymax);
+    if (!allnormal) { xmin = 1; ymin = 1; xmax = -1; ymax = -1; }
+
+    quad_store[(size_t)r * Q + (u + v * N)] = quad_lambert(p0, p1, p2, p3);
+
+    // ---- sample (reyes.cl:270-398) ----
+    const int32_t mintx = xmin >> (sh + 3), minty = ymin >> (sh + 3);
+    const int32_t maxtx = xmax >> (sh + 3), maxty = ymax >> (sh + 3);
+    if (mintx <= maxtx && minty <= maxty) {
+        const uint32_t id = (r - vw.range_base) * (uint32_t)Q + (uint32_t)(u + v * N);
+        unsigned long long* keys = vw.keys;
+        const int gw = vw.grid_w;
+        // pixel-candidate bounds with the reference's initialisers
+        const int32_t mingx = min(vmaxx + 1, qminx), mingy = min(vmaxy + 1, qminy);
+        const int32_t maxgx = max(-1, qmaxx), maxgy = max(-1, qmaxy);
+        // Exact-integer regime: all coordinates within +-2^18 and edges
+        // shorter than 2^11 sub-pixels keep every edge product inside
+        // int32, so quads that are not front-facing and pixels outside a
+        // quad's bounding box can never pass the edge test.
+        const int32_t lim = 1 << 18;
+        const bool small = (qminx > -lim) && (qmaxx < lim) && (qminy > -lim) && (qmaxy < lim) &&
+                           (qmaxx - qminx < (1 << 11)) && (qmaxy - qminy < (1 << 11));
+        if (__all(small)) {
+            if (ff) {
+                const int32_t X0 = max(mingx >> sh, mintx * 8), X1 = min(maxgx >> sh, maxtx * 8 + 7);
+                const int32_t Y0 = max(mingy >> sh, minty * 8), Y1 = min(maxgy >> sh, maxty * 8 + 7);
+                for (int32_t Y = Y0; Y <= Y1; ++Y) {
+                    for (int32_t X = X0; X <= X1; ++X) {
+                        const int32_t tx = X << sh, ty = Y << sh;
+                        float dd = 1.0f;
+                        const bool in1 = inside_tri(x0.x, x0.y, x1.x, x1.y, x3.x, x3.y, d0, d1, d3, tx, ty, dd);
+                        const bool in2 = inside_tri(x0.x, x0.y, x3.x, x3.y, x2.x, x2.y, d0, d3, d2, tx, ty, dd);
+                        if (in1 || in2) {
+                            const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
+                            key_min(&keys[fb_index(X, Y, gw)], key);
+                            ++natom;
+                        }
+                    }
+                }
+            }
+        } else {
+            // literal per-tile loops of the reference (wide coordinates)
+            if (lane == 0) ++slow;
+            const int32_t maxl = (8 << sh) - 1;
+            for (int32_t ty = minty; ty <= maxty; ++ty) {
+                for (int32_t tx = mintx; tx <= maxtx; ++tx) {
+                    const int32_t osx = (tx * 8) << sh, osy = (ty * 8) << sh;
+                    const int32_t minpx = min(max(wsub(mingx, osx), 0), maxl) >> sh;
+                    const int32_t minpy = min(max(wsub(mingy, osy), 0), maxl) >> sh;
+                    const int32_t maxpx = min(max(wsub(maxgx, osx), 0), maxl) >> sh;
+                    const int32_t maxpy = min(max(wsub(maxgy, osy), 0), maxl) >> sh;
+                    for (int32_t y = minpy; y <= maxpy; ++y) {
+                        for (int32_t x = minpx; x <= maxpx; ++x) {
+                            const int32_t tpx = (x << sh) + osx, tpy = (y << sh) + osy;
+                            float dd = 1.0f;
+                            const bool in1 = inside_tri(x0.x, x0.y, x1.x, x1.y, x3.x, x3.y, d0, d1, d3, tpx, tpy, dd);
+                            const bool in2 = inside_tri(x0.x, x0.y, x3.x, x3.y, x2.x, x2.y, d0, d3, d2, tpx, tpy, dd);
+                            if (in1 || in2) {
+                                const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
+                                key_min(&keys[fb_index(tx * 8 + x, ty * 8 + y, gw)], key);
+                                ++natom;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// NT threads per work-group: N*N (one wave per 8x8 block of quads) or 64
+// (one wave dices the whole grid and shades the blocks in turn: no waiting
+// between waves at the range's barriers).
+template <int N, int MODE, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_raster(const RangeRec* __restrict__ canon,
                                                   const DevObject* __restrict__ objs,
                                                   const DevView* __restrict__ views,
                                                   DevParams prm, FrameCounters* fc,
@@ -1253,21 +1372,20 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     constexpr int Q = N * N;
     constexpr int B = N / 8;
     __shared__ float4 s_cpb[2][20];   // control points: current range / next range
-    __shared__ float4 s_pos[G];
+    // eye-space x, y, z per grid point (the LDS-cached noise path also keeps
+    // the object-space w there between its passes)
+    typedef typename std::conditional<MODE == 2, float4, EyePos>::type PosT;
+    __shared__ PosT s_pos[G];
     __shared__ int2 s_pxl[G];
     __shared__ float s_dep[G];
     __shared__ float4 s_nrm[G];
-    __shared__ float s_box[Q / 64][6];
+    __shared__ float s_box[NT / 64][6];
     __shared__ int s_geo[kCachedOctaves][6];
     __shared__ float s_lat[kCachedOctaves][kCacheEntries];
     __shared__ float4 s_col[MODE == 1 ? G1 * 8 : 1];   // normal column sums per u column
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int u = (wave % B) * 8 + (lane & 7);
-    const int v = (wave / B) * 8 + (lane >> 3);
-    const int g0 = u + v * G1;
-    const int sh = prm.subpixel_bits;
     const uint32_t total = fc->total_ranges;
     uint32_t slow = 0, natom = 0, uncached = 0;
     __shared__ uint32_t s_next;
@@ -1330,7 +1448,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
         const bool disp_cols = MODE == 1 && ob.type == MP_TYPE_BEZIER;
         if (disp_cols) {
             // displacement normal: column sums once per u column
-            for (int t = tid; t < G1 * 8; t += Q) {
+            for (int t = tid; t < G1 * 8; t += NT) {
                 const int c = t >> 3;
                 const float uc = rec.umin + (rec.umax - rec.umin) * ((float)c / (float)N);
                 s_col[t] = normal_column(s_cp, uc, t & 7);
@@ -1341,7 +1459,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
             // wave-uniform trip count with every lane active (the lane-window
             // noise exchanges values across the wave); lanes past the last
             // point repeat it and store nothing
-            for (int gb = wave * 64; gb < G; gb += Q) {
+            for (int gb = wave * 64; gb < G; gb += NT) {
                 const int g = min(gb + lane, G - 1);
                 const int nu = g % G1, nv = g / G1;
                 const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
@@ -1356,17 +1474,17 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
                 const int32_t px = f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2));
                 const int32_t py = f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2));
                 if (gb + lane < G) {
-                    s_pos[g] = make_float4(e[0], e[1], e[2], e[3]);
+                    set_pos(s_pos[g], e[0], e[1], e[2], e[3]);
                     s_pxl[g] = make_int2(px, py);
                     s_dep[g] = q[2] / q[3];
                 }
             }
-        } else {
+        } else if constexpr (MODE == 2) {
             // displaced: surface point and normal, then the value noise with
             // each octave's lattice values hashed once per range into LDS
             float bx0 = INFINITY, by0 = INFINITY, bz0 = INFINITY;
             float bx1 = -INFINITY, by1 = -INFINITY, bz1 = -INFINITY;
-            for (int g = tid; g < G; g += Q) {
+            for (int g = tid; g < G; g += NT) {
                 const int nu = g % G1, nv = g / G1;
                 const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
                 const float uu = rec.umin + (rec.umax - rec.umin) * tu;
@@ -1394,7 +1512,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
             if (tid < noct) {
                 float m[6];
                 for (int k = 0; k < 6; ++k) m[k] = s_box[0][k];
-                for (int w = 1; w < Q / 64; ++w)
+                for (int w = 1; w < NT / 64; ++w)
                     for (int k = 0; k < 3; ++k) {
                         m[k] = fminf(m[k], s_box[w][k]);
                         m[k + 3] = fmaxf(m[k + 3], s_box[w][k + 3]);
@@ -1418,14 +1536,14 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
             for (int k = 0; k < noct; ++k) {
                 const int nx = s_geo[k][3], ny = s_geo[k][4], nzc = s_geo[k][5];
                 const int cnt = nx * ny * nzc;
-                for (int i = tid; i < cnt; i += Q) {
+                for (int i = tid; i < cnt; i += NT) {
                     const int x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
                     s_lat[k][i] = lattice(s_geo[k][0] + x, s_geo[k][1] + y, s_geo[k][2] + z,
                                           prm.disp_seed + (uint32_t)k);
                 }
             }
             __syncthreads();
-            for (int g = tid; g < G; g += Q) {
+            for (int g = tid; g < G; g += NT) {
                 const float4 p4 = s_pos[g];
                 const float4 n4 = s_nrm[g];
                 float acc = 0.0f, amp = 1.0f, norm = 0.0f, freq = prm.disp_frequency;
@@ -1450,7 +1568,7 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
                 float e[4], q[4];
                 mat4_mul(ob.mv, P, e);
                 mat4_mul(vw.proj, e, q);
-                s_pos[g] = make_float4(e[0], e[1], e[2], e[3]);
+                set_pos(s_pos[g], e[0], e[1], e[2], e[3]);
                 const int32_t px = f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2));
                 const int32_t py = f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2));
                 s_pxl[g] = make_int2(px, py);
@@ -1465,97 +1583,13 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
         // is in order, so a later wait would also wait for the atomics).
         if (have_pf) s_cpb[buf ^ 1][tid] = pf;
 
-        // ---- shade (reyes.cl:84-199); one wave = one 8x8 block ----
-        const float4 p0 = s_pos[g0], p1 = s_pos[g0 + 1], p2 = s_pos[g0 + G1], p3 = s_pos[g0 + G1 + 1];
-        const int2 x0 = s_pxl[g0], x1 = s_pxl[g0 + 1], x2 = s_pxl[g0 + G1], x3 = s_pxl[g0 + G1 + 1];
-        const float d0 = s_dep[g0], d1 = s_dep[g0 + 1], d2 = s_dep[g0 + G1], d3 = s_dep[g0 + G1 + 1];
-        const bool zero = (p0.z == 0.0f) || (p1.z == 0.0f) || (p2.z == 0.0f) || (p3.z == 0.0f);
-        const bool allnormal = !__any(zero);
-        const int32_t vmaxx = (vw.width << sh) - 1, vmaxy = (vw.height << sh) - 1;
-        bool ff = true;
-        if (prm.backface_culling) {
-            const int32_t d1x = wsub(x1.x, x0.x), d1y = wsub(x1.y, x0.y);
-            const int32_t d2x = wsub(x2.x, x0.x), d2y = wsub(x2.y, x0.y);
-            const int32_t d3x = wsub(x3.x, x0.x), d3y = wsub(x3.y, x0.y);
-            ff = (wsub(wmul(d1x, d3y), wmul(d3x, d1y)) < 0) || (wsub(wmul(d3x, d2y), wmul(d2x, d3y)) < 0);
+        // one wave per 8x8 block; a work-group of fewer waves takes several
+        for (int blk = wave; blk < Q / 64; blk += NT / 64) {
+            const int u = (blk % B) * 8 + (lane & 7);
+            const int v = (blk / B) * 8 + (lane >> 3);
+            shade_sample<N>(s_pos, s_pxl, s_dep, u, v, lane, vw, prm, r, quad_store, slow, natom);
         }
-        const int32_t qminx = min(min(x0.x, x1.x), min(x2.x, x3.x));
-        const int32_t qminy = min(min(x0.y, x1.y), min(x2.y, x3.y));
-        const int32_t qmaxx = max(max(x0.x, x1.x), max(x2.x, x3.x));
-        const int32_t qmaxy = max(max(x0.y, x1.y), max(x2.y, x3.y));
-        int32_t xmin = wave_min_i(ff ? min(vmaxx, qminx) : vmaxx);
-        int32_t ymin = wave_min_i(ff ? min(vmaxy, qminy) : vmaxy);
-        int32_t xmax = wave_max_i(ff ? max(0, qmaxx) : 0);
-        int32_t ymax = wave_max_i(ff ? max(0, qmaxy) : 0);
-        xmin = max(0, xmin); ymin = max(0, ymin);
-        xmax = min(vmaxx, xmax); ymax = min(vmaxy, ymax);
-        if (!allnormal) { xmin = 1; ymin = 1; xmax = -1; ymax = -1; }
 
-        quad_store[(size_t)r * Q + (u + v * N)] = quad_lambert(p0, p1, p2, p3);
-
-        // ---- sample (reyes.cl:270-398) ----
-        const int32_t mintx = xmin >> (sh + 3), minty = ymin >> (sh + 3);
-        const int32_t maxtx = xmax >> (sh + 3), maxty = ymax >> (sh + 3);
-        if (mintx <= maxtx && minty <= maxty) {
-            const uint32_t id = (r - vw.range_base) * (uint32_t)Q + (uint32_t)(u + v * N);
-            unsigned long long* keys = vw.keys;
-            const int gw = vw.grid_w;
-            // pixel-candidate bounds with the reference's initialisers
-            const int32_t mingx = min(vmaxx + 1, qminx), mingy = min(vmaxy + 1, qminy);
-            const int32_t maxgx = max(-1, qmaxx), maxgy = max(-1, qmaxy);
-            // Exact-integer regime: all coordinates within +-2^18 and edges
-            // shorter than 2^11 sub-pixels keep every edge product inside
-            // int32, so quads that are not front-facing and pixels outside a
-            // quad's bounding box can never pass the edge test.
-            const int32_t lim = 1 << 18;
-            const bool small = (qminx > -lim) && (qmaxx < lim) && (qminy > -lim) && (qmaxy < lim) &&
-                               (qmaxx - qminx < (1 << 11)) && (qmaxy - qminy < (1 << 11));
-            if (__all(small)) {
-                if (ff) {
-                    const int32_t X0 = max(mingx >> sh, mintx * 8), X1 = min(maxgx >> sh, maxtx * 8 + 7);
-                    const int32_t Y0 = max(mingy >> sh, minty * 8), Y1 = min(maxgy >> sh, maxty * 8 + 7);
-                    for (int32_t Y = Y0; Y <= Y1; ++Y) {
-                        for (int32_t X = X0; X <= X1; ++X) {
-                            const int32_t tx = X << sh, ty = Y << sh;
-                            float dd = 1.0f;
-                            const bool in1 = inside_tri(x0.x, x0.y, x1.x, x1.y, x3.x, x3.y, d0, d1, d3, tx, ty, dd);
-                            const bool in2 = inside_tri(x0.x, x0.y, x3.x, x3.y, x2.x, x2.y, d0, d3, d2, tx, ty, dd);
-                            if (in1 || in2) {
-                                const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
-                                key_min(&keys[fb_index(X, Y, gw)], key);
-                                ++natom;
-                            }
-                        }
-                    }
-                }
-            } else {
-                // literal per-tile loops of the reference (wide coordinates)
-                if (lane == 0) ++slow;
-                const int32_t maxl = (8 << sh) - 1;
-                for (int32_t ty = minty; ty <= maxty; ++ty) {
-                    for (int32_t tx = mintx; tx <= maxtx; ++tx) {
-                        const int32_t osx = (tx * 8) << sh, osy = (ty * 8) << sh;
-                        const int32_t minpx = min(max(wsub(mingx, osx), 0), maxl) >> sh;
-                        const int32_t minpy = min(max(wsub(mingy, osy), 0), maxl) >> sh;
-                        const int32_t maxpx = min(max(wsub(maxgx, osx), 0), maxl) >> sh;
-                        const int32_t maxpy = min(max(wsub(maxgy, osy), 0), maxl) >> sh;
-                        for (int32_t y = minpy; y <= maxpy; ++y) {
-                            for (int32_t x = minpx; x <= maxpx; ++x) {
-                                const int32_t tpx = (x << sh) + osx, tpy = (y << sh) + osy;
-                                float dd = 1.0f;
-                                const bool in1 = inside_tri(x0.x, x0.y, x1.x, x1.y, x3.x, x3.y, d0, d1, d3, tpx, tpy, dd);
-                                const bool in2 = inside_tri(x0.x, x0.y, x3.x, x3.y, x2.x, x2.y, d0, d3, d2, tpx, tpy, dd);
-                                if (in1 || in2) {
-                                    const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
-                                    key_min(&keys[fb_index(tx * 8 + x, ty * 8 + y, gw)], key);
-                                    ++natom;
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-        }
         buf ^= 1;
         r = rn;
         r_end = rn_end;
@@ -1567,11 +1601,11 @@ __global__ __launch_bounds__(N * N) void k_raster(const RangeRec* __restrict__ c
     if (lane == 0 && natom) atomicAdd(&fc->atomics, (unsigned long long)natom);
 }
 
-template <int N, int MODE>
+template <int N, int MODE, int NT>
 static int raster_occupancy()
 {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_raster<N, MODE>, N * N, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_raster<N, MODE, NT>, NT, 0) != hipSuccess)
         blocks = 1;
     return blocks < 1 ? 1 : blocks;
 }
@@ -1585,30 +1619,32 @@ int raster_grid(int patch_size, int num_cus, const DevParams& prm)
 {
     const int mode = raster_mode(prm);
     int per_cu = 1;
-#define MP_OCC(NN)                                                                 \
-    per_cu = mode == 0 ? raster_occupancy<NN, 0>() : (mode == 1 ? raster_occupancy<NN, 1>() \
-                                                                : raster_occupancy<NN, 2>());
+#define MP_OCC(NN, NT)                                                                     \
+    per_cu = mode == 0 ? raster_occupancy<NN, 0, NT>() : (mode == 1 ? raster_occupancy<NN, 1, NT>() \
+                                                                    : raster_occupancy<NN, 2, NT>());
     switch (patch_size) {
-    case 8: MP_OCC(8) break;
-    case 16: MP_OCC(16) break;
-    case 32: MP_OCC(32) break;
+    case 8: MP_OCC(8, 64) break;
+    case 16:
+        if (prm.raster_one_wave) { MP_OCC(16, 64) } else { MP_OCC(16, 256) }
+        break;
+    case 32: MP_OCC(32, 1024) break;
     default: break;
     }
 #undef MP_OCC
     return per_cu * num_cus;
 }
 
-template <int N>
+template <int N, int NT>
 static void launch_raster_n(int mode, int grid, hipStream_t s, const RangeRec* canon,
                             const DevObject* objs, const DevView* views, const DevParams& prm,
                             FrameCounters* fc, float* quad_store)
 {
     if (mode == 0)
-        hipLaunchKernelGGL((k_raster<N, 0>), dim3(grid), dim3(N * N), 0, s, canon, objs, views, prm, fc, quad_store);
+        hipLaunchKernelGGL((k_raster<N, 0, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store);
     else if (mode == 1)
-        hipLaunchKernelGGL((k_raster<N, 1>), dim3(grid), dim3(N * N), 0, s, canon, objs, views, prm, fc, quad_store);
+        hipLaunchKernelGGL((k_raster<N, 1, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store);
     else
-        hipLaunchKernelGGL((k_raster<N, 2>), dim3(grid), dim3(N * N), 0, s, canon, objs, views, prm, fc, quad_store);
+        hipLaunchKernelGGL((k_raster<N, 2, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store);
 }
 
 hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject* objs,
@@ -1617,9 +1653,12 @@ hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject*
 {
     const int mode = raster_mode(prm);
     switch (patch_size) {
-    case 8: launch_raster_n<8>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
-    case 16: launch_raster_n<16>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
-    case 32: launch_raster_n<32>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
+    case 8: launch_raster_n<8, 64>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
+    case 16:
+        if (prm.raster_one_wave) launch_raster_n<16, 64>(mode, grid, s, canon, objs, views, prm, fc, quad_store);
+        else launch_raster_n<16, 256>(mode, grid, s, canon, objs, views, prm, fc, quad_store);
+        break;
+    case 32: launch_raster_n<32, 1024>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
