@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default scheduling (profiling runs: one kernel configuration)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
     ap.add_argument("--adjoint-only", type=int, default=0, help=argparse.SUPPRESS)
@@ -159,7 +161,7 @@ def main():
     st = r.stats()
     use_graph = not args.no_graph
     # pick the fastest scheduling of this frame (output unchanged)
-    tuning = r.autotune(5) if use_graph else r.tuning()
+    tuning = r.autotune(5) if use_graph and not args.no_autotune else r.tuning()
     print("autotuned scheduling: %s" % tuning, file=sys.stderr, flush=True)
     print("first frame: %d ranges, %d micropolygons, %d coverage samples" % (
         st["diced_ranges"], st["micropolygons"], st["coverage_samples"]), file=sys.stderr, flush=True)
