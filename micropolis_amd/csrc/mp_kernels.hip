@@ -1226,14 +1226,22 @@ MP_DEV float quad_lambert(const float4& p0, const float4& p1, const float4& p2, 
     normalize3(nx, ny, nzz);
     float lx = 4.0f, ly = 3.0f, lz = 8.0f;
     normalize3(lx, ly, lz);
+    const float s = cl_max(dot3(nx, ny, nzz, lx, ly, lz), 0.0f);
     float sx = ((p0.x + p1.x) + p2.x) + p3.x;
     float sy = ((p0.y + p1.y) + p2.y) + p3.y;
     float sz = ((p0.z + p1.z) + p2.z) + p3.z;
+    // The specular term only matters when it is NaN.  With n finite, a
+    // position sum in front of the eye (z < 0) and away from under- and
+    // overflow, the unit sum has z < 0, l - unit sum has z > l.z > 0, so the
+    // half vector and its dot product with n are finite: the result is s.
+    const float lo = 1.0e-18f, hi = 1.0e18f;
+    if (fabsf(nx) < INFINITY && fabsf(ny) < INFINITY && fabsf(nzz) < INFINITY && sz < -lo && sz > -hi &&
+        fabsf(sx) < hi && fabsf(sy) < hi)
+        return s;
     normalize3(sx, sy, sz);
     float hx = lx + (-sx), hy = ly + (-sy), hz = lz + (-sz);
     normalize3(hx, hy, hz);
     const float spec = cl_max(dot3(nx, ny, nzz, hx, hy, hz), 0.0f);
-    const float s = cl_max(dot3(nx, ny, nzz, lx, ly, lz), 0.0f);
     return (spec != spec) ? spec : s;
 }
 
@@ -1311,8 +1319,13 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
                            (qmaxx - qminx < (1 << 11)) && (qmaxy - qminy < (1 << 11));
         if (__all(small)) {
             if (ff) {
-                const int32_t X0 = max(mingx >> sh, mintx * 8), X1 = min(maxgx >> sh, maxtx * 8 + 7);
-                const int32_t Y0 = max(mingy >> sh, minty * 8), Y1 = min(maxgy >> sh, maxty * 8 + 7);
+                // pixel X samples at its corner X << sh, inside the box only
+                // for X >= ceil(mingx / 2^sh); the reference's loop starts at
+                // the floor, whose corner lies outside the box (never passes)
+                // unless mingx is a whole pixel
+                const int32_t rnd = (1 << sh) - 1;
+                const int32_t X0 = max((mingx + rnd) >> sh, mintx * 8), X1 = min(maxgx >> sh, maxtx * 8 + 7);
+                const int32_t Y0 = max((mingy + rnd) >> sh, minty * 8), Y1 = min(maxgy >> sh, maxty * 8 + 7);
                 for (int32_t Y = Y0; Y <= Y1; ++Y) {
                     for (int32_t X = X0; X <= X1; ++X) {
                         const int32_t tx = X << sh, ty = Y << sh;
