@@ -148,6 +148,15 @@ int mp_finish(mp_context* ctx);
 int mp_read_framebuffer(mp_context* ctx, uint8_t* rgba8, float* depth,
                         uint32_t* ids, float* rgba32);
 
+/* Display conversion of the rgba8 readback (gamma 1/2.2 of
+ * shaders/tex_draw.frag:25-37, computed with the host's powf and rounded to
+ * 8 bits; no reference counterpart as a call).  thresholds[k-1] = the
+ * smallest linear value whose byte is at least k, k = 1..255; a value's byte
+ * is the number of thresholds <= it (0 for NaN and values <= 0).  The
+ * resolve kernel converts with this table, so rgba8 equals the host
+ * conversion bit for bit.  Host only. */
+int mp_display_thresholds(float thresholds[255]);
+
 /* Debug tap: DRAW ranges of the last frame in canonical order, as
  * (pid, object, depth, umin, vmin, umax, vmax) records; *count receives the
  * number of ranges.  Pass records == NULL to query the count. */

@@ -100,6 +100,7 @@ EXPORTS = [
     ("mp_replay", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     ("mp_replay_stages", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
     ("mp_measure_atomic_rate", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double)]),
+    ("mp_display_thresholds", C.c_int, [C.POINTER(C.c_float)]),
     ("mp_backward", C.c_int, [_P, _P, _P, _P, _P]),
     ("mp_batch_read_device", C.c_int, [_P, C.c_int32, _P, _P]),
     ("mp_update_patches_device", C.c_int, [_P, C.c_uint64, _P, C.c_size_t]),

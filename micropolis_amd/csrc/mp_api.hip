@@ -112,6 +112,7 @@ struct mp_context {
     uint32_t* d_part = nullptr;
     uint32_t gpid_cap = 0;
     float* d_quad = nullptr; size_t quad_cap = 0;
+    float* d_display = nullptr;  // display byte thresholds (mp_display_thresholds)
 
     // last frame
     uint32_t n_gpids = 0;
@@ -300,6 +301,45 @@ extern "C" int mp_config_defaults(mp_config* c)
 
 extern "C" const char* mp_create_error(void) { return g_create_error.c_str(); }
 
+// Display byte of a linear colour value: gamma 1/2.2 (shaders/tex_draw.frag:25-37)
+// with the host's powf, rounded to 8 bits, 0 for NaN and values <= 0.
+static uint32_t display_byte_host(float c)
+{
+    if (!(c > 0.0f)) return 0;
+    const float g = powf(c, 0.454545f);
+    if (!(g < 1.0f)) return 255;
+    return (uint32_t)(g * 255.0f + 0.5f);
+}
+
+// tab[k - 1] = smallest positive float whose display byte is at least k
+// (binary search over the bit patterns of positive floats, on which the
+// byte is non-decreasing); tab[255] = NaN, never passed by a comparison.
+static void display_thresholds(float tab[256])
+{
+    for (uint32_t k = 1; k <= 255; ++k) {
+        uint32_t lo = 1u, hi = 0x7f800000u;   // smallest denormal .. +inf (byte 255)
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            float f;
+            memcpy(&f, &mid, 4);
+            if (display_byte_host(f) >= k) hi = mid;
+            else lo = mid + 1;
+        }
+        memcpy(&tab[k - 1], &lo, 4);
+    }
+    const uint32_t qnan = 0x7fc00000u;
+    memcpy(&tab[255], &qnan, 4);
+}
+
+extern "C" int mp_display_thresholds(float thresholds[255])
+{
+    if (!thresholds) return MP_E_INVALID;
+    float tab[256];
+    display_thresholds(tab);
+    memcpy(thresholds, tab, 255 * sizeof(float));
+    return MP_OK;
+}
+
 extern "C" mp_context* mp_create(const mp_config* cfg)
 {
     if (!cfg) { g_create_error = "null config"; return nullptr; }
@@ -333,10 +373,20 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
         hipMalloc((void**)&ctx->d_fc, sizeof(FrameCounters)) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_fc, sizeof(FrameCounters), 0) != hipSuccess ||
         hipMalloc((void**)&ctx->d_totals, 4 * sizeof(uint32_t)) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_totals, 4 * sizeof(uint32_t), 0) != hipSuccess) {
+        hipHostMalloc((void**)&ctx->h_totals, 4 * sizeof(uint32_t), 0) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_display, 256 * sizeof(float)) != hipSuccess) {
         g_create_error = "HIP initialisation failed";
         mp_destroy(ctx);
         return nullptr;
+    }
+    {
+        float tab[256];
+        display_thresholds(tab);
+        if (hipMemcpy(ctx->d_display, tab, sizeof(tab), hipMemcpyHostToDevice) != hipSuccess) {
+            g_create_error = "HIP initialisation failed";
+            mp_destroy(ctx);
+            return nullptr;
+        }
     }
     ctx->num_cus = prop.multiProcessorCount;
     // hashing the lattice per use measured faster than the per-range LDS cache
@@ -392,6 +442,7 @@ extern "C" void mp_destroy(mp_context* ctx)
     hipFree(ctx->d_objs); hipHostFree(ctx->h_objs);
     hipFree(ctx->d_fc); hipHostFree(ctx->h_fc);
     hipFree(ctx->d_totals); hipHostFree(ctx->h_totals);
+    hipFree(ctx->d_display);
     hipFree(ctx->d_queue); hipFree(ctx->d_queue_cnt); hipFree(ctx->d_shared_q); hipFree(ctx->d_shared_cnt);
     hipFree(ctx->d_chunks); hipFree(ctx->d_chunk_count); hipFree(ctx->d_chunk_ready); hipFree(ctx->d_share);
     hipFree(ctx->d_keys); hipFree(ctx->d_span);
@@ -820,7 +871,7 @@ static int launch_back_end(mp_context* ctx, hipEvent_t* ev = nullptr)
     if (!ctx->cfg.dummy_render)
         HIPCHK(ctx, launch_resolve(ctx->d_views, ctx->n_views, ctx->max_pixels, ctx->d_canon, ctx->d_objs,
                                    ctx->d_quad, ctx->cfg.reyes_patch_size, ctx->cfg.write_rgba32,
-                                   ctx->stream));
+                                   ctx->d_display, ctx->stream));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[2], ctx->stream));
     return MP_OK;
 }

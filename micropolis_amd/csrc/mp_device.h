@@ -577,6 +577,61 @@ MP_DEV bool inside_tri(int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t x
     return ok;
 }
 
+// inside_tri in the exact-integer regime (every vertex and sample within
+// +-2^18 sub-pixels, every edge shorter than 2^11): each product has 24-bit
+// operands and a result below 2^30, so v_mul_i32_i24 and plain adds give the
+// wrapping arithmetic's values, and the three barycentric denominators are
+// one integer (twice the signed area: edge j at its opposite vertex).  Set
+// up once per triangle, tested per pixel.
+struct SmallTri {
+    int32_t dx[3], dy[3], o[3], c[3];
+    float den, rden;     // denominator and its Newton-refined reciprocal
+};
+
+MP_DEV void small_tri_setup(SmallTri& t, int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t x2, int32_t y2)
+{
+    const int32_t Px[3] = { x0, x1, x2 }, Py[3] = { y0, y1, y2 };
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int j = (i + 1) % 3;
+        t.dx[i] = Py[j] - Py[i];
+        t.dy[i] = Px[i] - Px[j];
+        t.o[i] = __mul24(t.dx[i], Px[i]) + __mul24(t.dy[i], Py[i]);
+        t.c[i] = (t.dx[i] > 0 || (t.dx[i] == 0 && t.dy[i] > 0)) ? -1 : 0;
+    }
+    t.den = (float)(__mul24(t.dx[1], Px[0]) + __mul24(t.dy[1], Py[0]) - t.o[1]);
+    const float r0 = __builtin_amdgcn_rcpf(t.den);
+    t.rden = __builtin_fmaf(__builtin_fmaf(-t.den, r0, 1.0f), r0, r0);
+}
+
+// a / den for integers 0 <= a <= den < 2^24: the correction steps of the
+// IEEE f32 division sequence (v_div_scale / v_div_fmas / v_div_fixup leave
+// operands in this range unscaled and the quotient unchanged), with the
+// reciprocal shared by the three weights.
+MP_DEV float small_div(float a, const SmallTri& t)
+{
+    float q = a * t.rden;
+    q = __builtin_fmaf(__builtin_fmaf(-t.den, q, a), t.rden, q);
+    return __builtin_fmaf(__builtin_fmaf(-t.den, q, a), t.rden, q);
+}
+
+MP_DEV bool small_tri_test(const SmallTri& t, int32_t tx, int32_t ty, float d0, float d1, float d2, float& depth)
+{
+    int32_t V[3];
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        V[i] = __mul24(t.dx[i], tx) + __mul24(t.dy[i], ty) - t.o[i];
+        ok = ok && (V[i] > t.c[i]);
+    }
+    if (ok) {
+        const float w0 = small_div((float)V[1], t), w1 = small_div((float)V[2], t), w2 = small_div((float)V[0], t);
+        const float d = (d0 * w0 + d1 * w1) + d2 * w2;
+        if (d < depth) depth = d;
+    }
+    return ok;
+}
+
 MP_DEV int fb_index(int x, int y, int grid_w)
 {
     return (((x >> 3) + grid_w * (y >> 3)) << 6) + (x & 7) + ((y & 7) << 3);

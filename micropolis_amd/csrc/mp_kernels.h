@@ -96,7 +96,7 @@ hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject*
 int raster_grid(int patch_size, int num_cus, const DevParams& prm);
 hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           const RangeRec* canon, const DevObject* objs, const float* quad_store,
-                          int patch_size, int write_rgba32, hipStream_t s);
+                          int patch_size, int write_rgba32, const float* display_tab, hipStream_t s);
 
 hipError_t launch_init_roots(int obj, uint32_t n, RangeRec* out, hipStream_t s);
 

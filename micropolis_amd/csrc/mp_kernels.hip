@@ -1326,12 +1326,15 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
                 const int32_t rnd = (1 << sh) - 1;
                 const int32_t X0 = max((mingx + rnd) >> sh, mintx * 8), X1 = min(maxgx >> sh, maxtx * 8 + 7);
                 const int32_t Y0 = max((mingy + rnd) >> sh, minty * 8), Y1 = min(maxgy >> sh, maxty * 8 + 7);
+                SmallTri t1, t2;
+                small_tri_setup(t1, x0.x, x0.y, x1.x, x1.y, x3.x, x3.y);
+                small_tri_setup(t2, x0.x, x0.y, x3.x, x3.y, x2.x, x2.y);
                 for (int32_t Y = Y0; Y <= Y1; ++Y) {
                     for (int32_t X = X0; X <= X1; ++X) {
                         const int32_t tx = X << sh, ty = Y << sh;
                         float dd = 1.0f;
-                        const bool in1 = inside_tri(x0.x, x0.y, x1.x, x1.y, x3.x, x3.y, d0, d1, d3, tx, ty, dd);
-                        const bool in2 = inside_tri(x0.x, x0.y, x3.x, x3.y, x2.x, x2.y, d0, d3, d2, tx, ty, dd);
+                        const bool in1 = small_tri_test(t1, tx, ty, d0, d1, d3, dd);
+                        const bool in2 = small_tri_test(t2, tx, ty, d0, d3, d2, dd);
                         if (in1 || in2) {
                             const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
                             key_min(&keys[fb_index(X, Y, gw)], key);
@@ -1680,20 +1683,27 @@ hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject*
 // ------------------------------------------------------------------------
 // resolve: key -> display colour, FP32 depth, id; clears the key
 
-MP_DEV uint8_t to_srgb8(float c)
+// Display byte of a linear value: tab[k - 1] is the smallest value whose
+// byte is at least k (k = 1..255, non-decreasing), so the byte is the number
+// of entries <= c; NaN and values <= 0 give 0.  Eight halving steps.
+MP_DEV uint32_t display_byte(float c, const float* tab)
 {
-    if (!(c > 0.0f)) return 0;
-    const float g = powf(c, 0.454545f);
-    if (!(g < 1.0f)) return 255;
-    return (uint8_t)(g * 255.0f + 0.5f);
+    uint32_t k = 0;
+#pragma unroll
+    for (uint32_t s = 128; s > 0; s >>= 1)
+        if (c >= tab[k + s - 1]) k += s;
+    return k;
 }
 
 __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ views,
                                                  const RangeRec* __restrict__ canon,
                                                  const DevObject* __restrict__ objs,
                                                  const float* __restrict__ quad_store, int Q,
-                                                 int write_rgba32)
+                                                 int write_rgba32, const float* __restrict__ display_tab)
 {
+    __shared__ float s_tab[256];
+    if (threadIdx.x < 255) s_tab[threadIdx.x] = display_tab[threadIdx.x];
+    __syncthreads();
     const DevView& vw = views[blockIdx.y];
     const uint32_t n = (uint32_t)vw.grid_w * (uint32_t)vw.grid_h * 64u;
     const float ac[4] = { 0.015f, 0.015f, 0.015f, 1.0f };
@@ -1721,10 +1731,8 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
 #pragma unroll
             for (int k = 0; k < 4; ++k) c[k] = (ac[k] * ob.color[k] + s * ob.color[k]) + 0.0f;
         }
-        uchar4 px;
-        px.x = to_srgb8(c[0]); px.y = to_srgb8(c[1]); px.z = to_srgb8(c[2]); px.w = 255;
-        *gptr(reinterpret_cast<uint32_t*>(vw.rgba8) + o) =
-            (uint32_t)px.x | ((uint32_t)px.y << 8) | ((uint32_t)px.z << 16) | ((uint32_t)px.w << 24);
+        *gptr(reinterpret_cast<uint32_t*>(vw.rgba8) + o) = display_byte(c[0], s_tab) |
+            (display_byte(c[1], s_tab) << 8) | (display_byte(c[2], s_tab) << 16) | (255u << 24);
         gptr(vw.depth)[o] = d;
         gptr(vw.ids)[o] = id;
         if (write_rgba32) stg4(reinterpret_cast<float4*>(vw.rgba32) + o, make_float4(c[0], c[1], c[2], c[3]));
@@ -1733,7 +1741,7 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
 
 hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           const RangeRec* canon, const DevObject* objs, const float* quad_store,
-                          int patch_size, int write_rgba32, hipStream_t s)
+                          int patch_size, int write_rgba32, const float* display_tab, hipStream_t s)
 {
     // one key per thread: every pixel's gather chain (range, object,
     // Lambert term) is in flight at once
@@ -1741,7 +1749,7 @@ hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
     if (bx > 65535) bx = 65535;
     if (bx < 1) bx = 1;
     hipLaunchKernelGGL(k_resolve, dim3(bx, n_views), dim3(256), 0, s, views, canon, objs,
-                       quad_store, patch_size * patch_size, write_rgba32);
+                       quad_store, patch_size * patch_size, write_rgba32, display_tab);
     return hipGetLastError();
 }
 

@@ -1,5 +1,5 @@
 """GPU parity: the HIP library against the CPU oracle on the same seeded
-inputs (bit-exact split ranges and visible ids; depth 1 ulp; RGBA8 1/255)."""
+inputs (bit-exact split ranges and visible ids; depth 1 ulp; RGBA8 exact)."""
 import numpy as np
 import pytest
 
