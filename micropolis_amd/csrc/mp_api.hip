@@ -60,6 +60,16 @@ struct ViewBuffers {
     float* rgba32 = nullptr;
 };
 
+// Display byte of a linear colour value: gamma 1/2.2 (shaders/tex_draw.frag:25-37)
+// with the host's powf, rounded to 8 bits, 0 for NaN and values <= 0.
+static uint32_t display_byte_host(float c)
+{
+    if (!(c > 0.0f)) return 0;
+    const float g = powf(c, 0.454545f);
+    if (!(g < 1.0f)) return 255;
+    return (uint32_t)(g * 255.0f + 0.5f);
+}
+
 }  // namespace
 
 struct mp_context {
@@ -244,6 +254,8 @@ static void make_view(const mp_projection& pr, const float clear[4], int subpixe
     v.vs_y = H << subpixel_bits;
     for (int k = 0; k < 3; ++k) v.clear_linear[k] = powf(clear[k], 2.2f);
     v.clear_linear[3] = 1000.0f;
+    v.clear_rgba8 = display_byte_host(v.clear_linear[0]) | (display_byte_host(v.clear_linear[1]) << 8) |
+                    (display_byte_host(v.clear_linear[2]) << 16) | (255u << 24);
 }
 
 // ---------------------------------------------------------------------------
@@ -300,16 +312,6 @@ extern "C" int mp_config_defaults(mp_config* c)
 }
 
 extern "C" const char* mp_create_error(void) { return g_create_error.c_str(); }
-
-// Display byte of a linear colour value: gamma 1/2.2 (shaders/tex_draw.frag:25-37)
-// with the host's powf, rounded to 8 bits, 0 for NaN and values <= 0.
-static uint32_t display_byte_host(float c)
-{
-    if (!(c > 0.0f)) return 0;
-    const float g = powf(c, 0.454545f);
-    if (!(g < 1.0f)) return 255;
-    return (uint32_t)(g * 255.0f + 0.5f);
-}
 
 // tab[k - 1] = smallest positive float whose display byte is at least k
 // (binary search over the bit patterns of positive floats, on which the

@@ -1684,15 +1684,19 @@ hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject*
 // resolve: key -> display colour, FP32 depth, id; clears the key
 
 // Display byte of a linear value: tab[k - 1] is the smallest value whose
-// byte is at least k (k = 1..255, non-decreasing), so the byte is the number
-// of entries <= c; NaN and values <= 0 give 0.  Eight halving steps.
+// byte is at least k (k = 1..255, increasing), so the byte is the k with
+// tab[k - 1] <= c < tab[k]; NaN and values below tab[0] give 0.  The hardware
+// log2 / exp2 estimate the byte, and the two loops walk to the k the table
+// defines from any start (within one step in practice: usually no step).
 MP_DEV uint32_t display_byte(float c, const float* tab)
 {
-    uint32_t k = 0;
-#pragma unroll
-    for (uint32_t s = 128; s > 0; s >>= 1)
-        if (c >= tab[k + s - 1]) k += s;
-    return k;
+    if (!(c >= tab[0])) return 0;
+    if (c >= tab[254]) return 255;
+    const float g = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(c) * 0.454545f);
+    int k = min(max((int)(g * 255.0f + 0.5f), 1), 254);
+    while (c < tab[k - 1]) --k;    // stops at k >= 1: c >= tab[0]
+    while (c >= tab[k]) ++k;       // stops at k <= 254: c < tab[254]
+    return (uint32_t)k;
 }
 
 __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ views,
@@ -1718,7 +1722,7 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
         const size_t o = (size_t)Y * vw.width + X;
         float c[4];
         float d = 1.0f;
-        uint32_t id = 0xFFFFFFFFu;
+        uint32_t id = 0xFFFFFFFFu, bytes = vw.clear_rgba8;
         if (key == MP_EMPTY_KEY) {
             c[0] = vw.clear_linear[0]; c[1] = vw.clear_linear[1];
             c[2] = vw.clear_linear[2]; c[3] = vw.clear_linear[3];
@@ -1730,9 +1734,10 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
             const DevObject& ob = objs[canon[rr].obj_depth & 0xffffffu];
 #pragma unroll
             for (int k = 0; k < 4; ++k) c[k] = (ac[k] * ob.color[k] + s * ob.color[k]) + 0.0f;
+            bytes = display_byte(c[0], s_tab) | (display_byte(c[1], s_tab) << 8) |
+                    (display_byte(c[2], s_tab) << 16) | (255u << 24);
         }
-        *gptr(reinterpret_cast<uint32_t*>(vw.rgba8) + o) = display_byte(c[0], s_tab) |
-            (display_byte(c[1], s_tab) << 8) | (display_byte(c[2], s_tab) << 16) | (255u << 24);
+        *gptr(reinterpret_cast<uint32_t*>(vw.rgba8) + o) = bytes;
         gptr(vw.depth)[o] = d;
         gptr(vw.ids)[o] = id;
         if (write_rgba32) stg4(reinterpret_cast<float4*>(vw.rgba32) + o, make_float4(c[0], c[1], c[2], c[3]));

@@ -44,7 +44,7 @@ struct DevView {
     uint32_t range_count;        // device-written
     uint32_t gpid_end;           // one past the global patch index of its last object
     uint32_t pixel_offset;       // first pixel of this view in multi-view buffers
-    uint32_t pad_;
+    uint32_t clear_rgba8;        // display bytes of the clear colour (alpha 255)
 };
 
 // One draw_patches call: patch buffer + modelview + colour.
