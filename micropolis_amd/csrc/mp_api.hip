@@ -1656,7 +1656,8 @@ extern "C" int mp_backward(mp_context* ctx, const float* dl_dcolor, float* grad_
         if (ctx->patches[o.handle].type != MP_BEZIER) return fail(ctx, MP_E_INVALID, "mp_backward needs Bezier patches");
     int rc;
     if (ctx->list_cap > ctx->visible_cap) {
-        if ((rc = dev_alloc(ctx, &ctx->d_visible, ctx->list_cap))) return rc;
+        // per-range visibility flags, then the list of visible ranges
+        if ((rc = dev_alloc(ctx, &ctx->d_visible, 2 * (size_t)ctx->list_cap))) return rc;
         ctx->visible_cap = ctx->list_cap;
     }
     if (!ctx->d_next && (rc = dev_alloc(ctx, &ctx->d_next, 1))) return rc;
@@ -1670,8 +1671,8 @@ extern "C" int mp_backward(mp_context* ctx, const float* dl_dcolor, float* grad_
     HIPCHK(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(grad_amplitude), (size_t)no, 0u, ctx->stream));
     HIPCHK(ctx, launch_adjoint(ctx->cfg.reyes_patch_size, ctx->d_canon, ctx->d_objs, ctx->d_views, ctx->n_views,
                                ctx->max_pixels, ctx->prm, ctx->d_fc, ctx->d_quad, dl_dcolor, ctx->d_visible,
-                               ctx->d_next, grad_cps, grad_color, grad_amplitude, ctx->adjoint_grid_size,
-                               ctx->stream));
+                               ctx->d_visible + ctx->visible_cap, ctx->d_next, grad_cps, grad_color,
+                               grad_amplitude, ctx->adjoint_grid_size, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.0f;

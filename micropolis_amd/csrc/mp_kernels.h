@@ -104,8 +104,8 @@ hipError_t launch_init_roots(int obj, uint32_t n, RangeRec* out, hipStream_t s);
 hipError_t launch_adjoint(int patch_size, const RangeRec* canon, const DevObject* objs,
                           const DevView* views, int n_views, int max_pixels, const DevParams& prm,
                           const FrameCounters* fc, const float* quad_store, const float* dl_dcolor,
-                          uint32_t* visible, uint32_t* next, float* grad_cps, float* grad_color,
-                          float* grad_amp, int grid, hipStream_t s);
+                          uint32_t* visible, uint32_t* vis_list, uint32_t* vis_count, float* grad_cps,
+                          float* grad_color, float* grad_amp, int grid, hipStream_t s);
 int adjoint_grid(int patch_size, int num_cus);
 int level_grid(int num_cus, int lanes_per_range, int per_cu);
 hipError_t launch_split_levels(const DevObject* objs, int n_objs, const DevView* views, const DevParams& prm,

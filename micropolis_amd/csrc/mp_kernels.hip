@@ -1833,6 +1833,25 @@ __global__ __launch_bounds__(256) void k_adj_mark(const DevView* __restrict__ vi
     }
 }
 
+// list the ranges that own a visible micropolygon (any order): one counter
+// atomic per wave
+__global__ __launch_bounds__(256) void k_adj_compact(const uint32_t* __restrict__ visible,
+                                                     const FrameCounters* __restrict__ fc,
+                                                     uint32_t* __restrict__ list, uint32_t* __restrict__ count)
+{
+    const uint32_t total = fc->total_ranges;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t base = blockIdx.x * 256u; base < total; base += gridDim.x * 256u) {
+        const uint32_t r = base + threadIdx.x;
+        const bool v = r < total && visible[r] != 0u;
+        const uint64_t m = __ballot(v);
+        uint32_t slot = 0;
+        if (lane == 0 && m) slot = atomicAdd(count, (uint32_t)__popcll(m));
+        slot = (uint32_t)__shfl((int)slot, 0);
+        if (v) list[slot + lanes_below(m)] = r;
+    }
+}
+
 struct V3 { float x, y, z; };
 MP_DEV V3 mk(float x, float y, float z) { V3 r = { x, y, z }; return r; }
 MP_DEV V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
@@ -1841,7 +1860,12 @@ MP_DEV V3 operator*(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 MP_DEV float vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 MP_DEV V3 vcross(V3 a, V3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 
-// noise value and gradient (same lattice and octave sum as noise3)
+// noise value and gradient (same lattice and octave sum as noise3).
+// full_wave: every lane of the wave runs the call, so the lattice corners can
+// come from the lane window (one hash per lane per octave, window_corners);
+// otherwise, or where a cell falls outside the window, each lane hashes its
+// eight corners.
+template <bool full_wave>
 MP_DEV float noise3_grad(const DevParams& prm, float x, float y, float z, V3& grad)
 {
     float acc = 0.0f, amp = 1.0f, norm = 0.0f, freq = prm.disp_frequency;
@@ -1853,10 +1877,15 @@ MP_DEV float noise3_grad(const DevParams& prm, float x, float y, float z, V3& gr
         const int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
         const float tx = px - fx, ty = py - fy, tz = pz - fz;
         const float wx = fade(tx), wy = fade(ty), wz = fade(tz);
-        const float c000 = lattice(ix, iy, iz, seed),         c100 = lattice(ix + 1, iy, iz, seed);
-        const float c010 = lattice(ix, iy + 1, iz, seed),     c110 = lattice(ix + 1, iy + 1, iz, seed);
-        const float c001 = lattice(ix, iy, iz + 1, seed),     c101 = lattice(ix + 1, iy, iz + 1, seed);
-        const float c011 = lattice(ix, iy + 1, iz + 1, seed), c111 = lattice(ix + 1, iy + 1, iz + 1, seed);
+        float c[8];
+        if (!full_wave || !window_corners(ix, iy, iz, seed, c)) {
+            c[0] = lattice(ix, iy, iz, seed);         c[1] = lattice(ix + 1, iy, iz, seed);
+            c[2] = lattice(ix, iy + 1, iz, seed);     c[3] = lattice(ix + 1, iy + 1, iz, seed);
+            c[4] = lattice(ix, iy, iz + 1, seed);     c[5] = lattice(ix + 1, iy, iz + 1, seed);
+            c[6] = lattice(ix, iy + 1, iz + 1, seed); c[7] = lattice(ix + 1, iy + 1, iz + 1, seed);
+        }
+        const float c000 = c[0], c100 = c[1], c010 = c[2], c110 = c[3];
+        const float c001 = c[4], c101 = c[5], c011 = c[6], c111 = c[7];
         const float x00 = lerpf_(c000, c100, wx), x10 = lerpf_(c010, c110, wx);
         const float x01 = lerpf_(c001, c101, wx), x11 = lerpf_(c011, c111, wx);
         const float y0 = lerpf_(x00, x10, wy), y1 = lerpf_(x01, x11, wy);
@@ -1875,11 +1904,24 @@ MP_DEV float noise3_grad(const DevParams& prm, float x, float y, float z, V3& gr
     return acc / norm;
 }
 
-MP_DEV float wave_sum_f(float x)
+// Wave sums of 64 values at once: lane l ends with the sum over the wave of
+// value l.  Each step keeps half of a lane's values and adds its partner's
+// copy of that half: 63 shuffles in six batches of independent ones instead
+// of 64 chains of six dependent ones.
+MP_DEV float wave_sums64(float (&v)[64])
 {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    return x;
+    for (int o = 32; o >= 1; o >>= 1) {
+        const bool upper = (lane & o) != 0;
+#pragma unroll
+        for (int i = 0; i < o; ++i) {
+            const float keep = upper ? v[i + o] : v[i];
+            const float send = upper ? v[i] : v[i + o];
+            v[i] = keep + __shfl_xor(send, o);
+        }
+    }
+    return v[0];
 }
 
 template <int N>
@@ -1889,7 +1931,8 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
                                                    const FrameCounters* __restrict__ fc,
                                                    const float* __restrict__ quad_store,
                                                    const float* __restrict__ dl_dcolor,
-                                                   const uint32_t* __restrict__ visible, uint32_t* next,
+                                                   const uint32_t* __restrict__ vis_list,
+                                                   const uint32_t* __restrict__ vis_count,
                                                    float* __restrict__ grad_cps, float* __restrict__ grad_color,
                                                    float* __restrict__ grad_amp)
 {
@@ -1899,32 +1942,19 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
     __shared__ int2 s_pxl[G];
     __shared__ float s_ge[G][3];
     __shared__ float s_red[W][52];
-    __shared__ uint32_t s_next, s_next_end;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t total = fc->total_ranges;
     const int sh = prm.subpixel_bits;
     const V3 L = [] { float lx = 4.0f, ly = 3.0f, lz = 8.0f; const float inv = 1.0f / sqrtf(89.0f);
                       return mk(lx * inv, ly * inv, lz * inv); }();
 
-    const uint32_t chunk = max(1u, min(64u, total / (gridDim.x * 16u)));
-    uint32_t r = 0, r_end = 0;
-    for (;;) {
-        if (tid == 0) {
-            // next visible range: claim chunks of ranges, skip hidden ones
-            for (;;) {
-                while (r < r_end && !visible[r]) ++r;
-                if (r < r_end) break;
-                r = atomicAdd(next, chunk);
-                if (r >= total) break;
-                r_end = min(r + chunk, total);
-            }
-            s_next = r;
-            s_next_end = r_end;
-        }
-        __syncthreads();
-        r = s_next;
-        r_end = s_next_end;
-        if (r >= total) break;
+    // visible ranges by static stride over their list; the next one's index
+    // is loaded a range ahead
+    const uint32_t count = *vis_count;
+    uint32_t i = blockIdx.x;
+    uint32_t r_ahead = i < count ? vis_list[i] : 0u;
+    for (; i < count; i += gridDim.x) {
+        const uint32_t r = r_ahead;
+        if (i + gridDim.x < count) r_ahead = vis_list[i + gridDim.x];
         const RangeRec rec = canon[r];
         const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
         const DevView& vw = views[ob.view];
@@ -1936,9 +1966,13 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
         float nlen[PPT], nzv[PPT], dd[PPT];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
+            // every lane of a wave that owns any point runs (lanes past the
+            // last point repeat it and store nothing): the noise lattice
+            // comes from the lane window
             const int g = tid + k * Q;
-            if (g < G) {
-                const int nu = g % G1, nv = g / G1;
+            if ((tid & ~63) + k * Q < G) {
+                const int gc = min(g, G - 1);
+                const int nu = gc % G1, nv = gc / G1;
                 const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
                 const float uu = rec.umin + (rec.umax - rec.umin) * tu;
                 const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
@@ -1961,7 +1995,7 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
                 V3 gz = mk(0.f, 0.f, 0.f);
                 float nzz = 0.0f, d = 0.0f;
                 if (prm.displacement) {
-                    nzz = noise3_grad(prm, P[0], P[1], P[2], gz);
+                    nzz = noise3_grad<true>(prm, P[0], P[1], P[2], gz);
                     d = ob.disp_amplitude * nzz;
                 }
                 Pu[k] = pu; Pv[k] = pv; nh[k] = h; gn[k] = gz; nlen[k] = len; nzv[k] = nzz; dd[k] = d;
@@ -1972,10 +2006,12 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
                 float e[4], q[4];
                 mat4_mul(ob.mv, Pd, e);
                 mat4_mul(vw.proj, e, q);
-                s_e[g] = make_float4(e[0], e[1], e[2], e[3]);
-                s_pxl[g] = make_int2(f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2)),
-                                     f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2)));
-                s_ge[g][0] = s_ge[g][1] = s_ge[g][2] = 0.0f;
+                if (g < G) {
+                    s_e[g] = make_float4(e[0], e[1], e[2], e[3]);
+                    s_pxl[g] = make_int2(f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2)),
+                                         f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2)));
+                    s_ge[g][0] = s_ge[g][1] = s_ge[g][2] = 0.0f;
+                }
             }
         }
         __syncthreads();
@@ -1987,10 +2023,18 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
         float gs = 0.0f, ga0 = 0.0f, ga1 = 0.0f, ga2 = 0.0f;
         if (s_fwd == s_fwd) {
             const int2 x0 = s_pxl[g0], x1 = s_pxl[g0 + 1], x2 = s_pxl[g0 + G1], x3 = s_pxl[g0 + G1 + 1];
-            const int X0 = max(0, min(min(x0.x, x1.x), min(x2.x, x3.x)) >> sh);
-            const int X1 = min(vw.width - 1, max(max(x0.x, x1.x), max(x2.x, x3.x)) >> sh);
-            const int Y0 = max(0, min(min(x0.y, x1.y), min(x2.y, x3.y)) >> sh);
-            const int Y1 = min(vw.height - 1, max(max(x0.y, x1.y), max(x2.y, x3.y)) >> sh);
+            const int32_t qminx = min(min(x0.x, x1.x), min(x2.x, x3.x)), qmaxx = max(max(x0.x, x1.x), max(x2.x, x3.x));
+            const int32_t qminy = min(min(x0.y, x1.y), min(x2.y, x3.y)), qmaxy = max(max(x0.y, x1.y), max(x2.y, x3.y));
+            // in the sampler's exact-integer regime only pixel corners inside
+            // the quad's box can be covered: start at the first one
+            const int32_t lim = 1 << 18;
+            const bool small = (qminx > -lim) && (qmaxx < lim) && (qminy > -lim) && (qmaxy < lim) &&
+                               (qmaxx - qminx < (1 << 11)) && (qmaxy - qminy < (1 << 11));
+            const int32_t rnd = small ? (1 << sh) - 1 : 0;
+            const int X0 = max(0, (qminx + rnd) >> sh);
+            const int X1 = min(vw.width - 1, qmaxx >> sh);
+            const int Y0 = max(0, (qminy + rnd) >> sh);
+            const int Y1 = min(vw.height - 1, qmaxy >> sh);
             {
                 for (int Y = Y0; Y <= Y1; ++Y)
                     for (int X = X0; X <= X1; ++X) {
@@ -2074,15 +2118,17 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
                     gcp[3 * c + 2] += w * gP.z + wu * gPu.z + wv * gPv.z;
                 }
         }
-        // reduce over the work-group: waves by shuffles, then across waves
-#pragma unroll
-        for (int i = 0; i < 48; ++i) {
-            const float t = wave_sum_f(gcp[i]);
-            if (lane == 0) s_red[wave][i] = t;
-        }
+        // reduce over the work-group: the 52 sums of each wave at once (lane
+        // i ends with sum i), then across waves
         {
-            const float t0 = wave_sum_f(gamp), t1 = wave_sum_f(ga0), t2 = wave_sum_f(ga1), t3 = wave_sum_f(ga2);
-            if (lane == 0) { s_red[wave][48] = t0; s_red[wave][49] = t1; s_red[wave][50] = t2; s_red[wave][51] = t3; }
+            float red[64];
+#pragma unroll
+            for (int i = 0; i < 48; ++i) red[i] = gcp[i];
+            red[48] = gamp; red[49] = ga0; red[50] = ga1; red[51] = ga2;
+#pragma unroll
+            for (int i = 52; i < 64; ++i) red[i] = 0.0f;
+            const float t = wave_sums64(red);
+            if (lane < 52) s_red[wave][lane] = t;
         }
         __syncthreads();
         if (tid < 52) {
@@ -2094,7 +2140,6 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
                 else atomicAdd(&grad_color[ob.draw_index * 4 + (tid - 49)], t);
             }
         }
-        ++r;
         __syncthreads();
     }
 }
@@ -2112,8 +2157,8 @@ int adjoint_grid(int patch_size, int num_cus)
 hipError_t launch_adjoint(int patch_size, const RangeRec* canon, const DevObject* objs,
                           const DevView* views, int n_views, int max_pixels, const DevParams& prm,
                           const FrameCounters* fc, const float* quad_store, const float* dl_dcolor,
-                          uint32_t* visible, uint32_t* next, float* grad_cps, float* grad_color,
-                          float* grad_amp, int grid, hipStream_t s)
+                          uint32_t* visible, uint32_t* vis_list, uint32_t* vis_count, float* grad_cps,
+                          float* grad_color, float* grad_amp, int grid, hipStream_t s)
 {
     const int Q = patch_size * patch_size;
     int bx = (max_pixels + 255) / 256;
@@ -2121,12 +2166,14 @@ hipError_t launch_adjoint(int patch_size, const RangeRec* canon, const DevObject
     hipLaunchKernelGGL(k_adj_mark, dim3(bx, n_views), dim3(256), 0, s, views, Q, visible);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_adj_compact, dim3(1024), dim3(256), 0, s, visible, fc, vis_list, vis_count);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if (patch_size == 8)
         hipLaunchKernelGGL(k_adjoint<8>, dim3(grid), dim3(64), 0, s, canon, objs, views, prm, fc, quad_store,
-                           dl_dcolor, visible, next, grad_cps, grad_color, grad_amp);
+                           dl_dcolor, vis_list, vis_count, grad_cps, grad_color, grad_amp);
     else if (patch_size == 16)
         hipLaunchKernelGGL(k_adjoint<16>, dim3(grid), dim3(256), 0, s, canon, objs, views, prm, fc, quad_store,
-                           dl_dcolor, visible, next, grad_cps, grad_color, grad_amp);
+                           dl_dcolor, vis_list, vis_count, grad_cps, grad_color, grad_amp);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
