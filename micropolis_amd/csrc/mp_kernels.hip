@@ -1904,26 +1904,6 @@ MP_DEV float noise3_grad(const DevParams& prm, float x, float y, float z, V3& gr
     return acc / norm;
 }
 
-// Wave sums of 64 values at once: lane l ends with the sum over the wave of
-// value l.  Each step keeps half of a lane's values and adds its partner's
-// copy of that half: 63 shuffles in six batches of independent ones instead
-// of 64 chains of six dependent ones.
-MP_DEV float wave_sums64(float (&v)[64])
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        const bool upper = (lane & o) != 0;
-#pragma unroll
-        for (int i = 0; i < o; ++i) {
-            const float keep = upper ? v[i + o] : v[i];
-            const float send = upper ? v[i] : v[i + o];
-            v[i] = keep + __shfl_xor(send, o);
-        }
-    }
-    return v[0];
-}
-
 template <int N>
 __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ canon,
                                                    const DevObject* __restrict__ objs,
@@ -1941,7 +1921,11 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
     __shared__ float4 s_e[G];
     __shared__ int2 s_pxl[G];
     __shared__ float s_ge[G][3];
-    __shared__ float s_red[W][52];
+    __shared__ float s_red[W][4];
+    __shared__ float s_gp[G][9];          // dL/d(P, dP/du, dP/dv) per grid point
+    __shared__ float s_row[G1][4][6];     // per grid row and i: the u sums
+    __shared__ float s_wu[G1][8];         // a_i(u), a_i'(u) per grid column
+    __shared__ float s_wv[G1][8];         // b_j(v), b_j'(v) per grid row
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sh = prm.subpixel_bits;
     const V3 L = [] { float lx = 4.0f, ly = 3.0f, lz = 8.0f; const float inv = 1.0f / sqrtf(89.0f);
@@ -1959,6 +1943,18 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
         const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
         const DevView& vw = views[ob.view];
         if (tid < 16) s_cp[tid] = ldg4(ob.cps + (size_t)rec.pid * 16 + tid);
+        if (tid < G1) {
+            const float uu = rec.umin + (rec.umax - rec.umin) * ((float)tid / (float)N);
+            const float su = 1.0f - uu;
+            bern_u(uu, su, &s_wu[tid][0]);
+            dbern(uu, su, &s_wu[tid][4]);
+        } else if (tid >= 64 - G1 && tid < 64) {
+            const int nv = tid - (64 - G1);
+            const float vv = rec.vmin + (rec.vmax - rec.vmin) * ((float)nv / (float)N);
+            const float sv = 1.0f - vv;
+            bern_v(vv, sv, &s_wv[nv][0]);
+            dbern(vv, sv, &s_wv[nv][4]);
+        }
         __syncthreads();
 
         // forward state of the grid points owned by this thread
@@ -2076,67 +2072,74 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
         }
         __syncthreads();
 
-        // grid points: eye -> displaced -> surface -> control points
-        float gcp[48];
-#pragma unroll
-        for (int i = 0; i < 48; ++i) gcp[i] = 0.0f;
+        // grid points: eye -> displaced -> surface; the gradients with
+        // respect to the surface point and its two derivatives go to LDS
         float gamp = 0.0f;
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const int g = tid + k * Q;
             if (g >= G) continue;
             const V3 ge = mk(s_ge[g][0], s_ge[g][1], s_ge[g][2]);
-            if (ge.x == 0.0f && ge.y == 0.0f && ge.z == 0.0f) continue;
-            const float* M = ob.mv;
-            const V3 gPd = mk(M[0] * ge.x + M[1] * ge.y + M[2] * ge.z, M[4] * ge.x + M[5] * ge.y + M[6] * ge.z,
-                              M[8] * ge.x + M[9] * ge.y + M[10] * ge.z);
-            V3 gP = gPd, gPu = mk(0.f, 0.f, 0.f), gPv = mk(0.f, 0.f, 0.f);
-            if (prm.displacement) {
-                const float nd = vdot(nh[k], gPd);
-                gamp += nzv[k] * nd;
-                gP = gP + gn[k] * (ob.disp_amplitude * nd);
-                if (nlen[k] > 0.0f) {
-                    const V3 gnh = gPd * dd[k];
-                    const V3 gN = (gnh - nh[k] * vdot(nh[k], gnh)) * (1.0f / nlen[k]);
-                    gPv = vcross(Pu[k], gN);
-                    gPu = vcross(gN, Pv[k]);
+            V3 gP = mk(0.f, 0.f, 0.f), gPu = mk(0.f, 0.f, 0.f), gPv = mk(0.f, 0.f, 0.f);
+            if (ge.x != 0.0f || ge.y != 0.0f || ge.z != 0.0f) {
+                const float* M = ob.mv;
+                const V3 gPd = mk(M[0] * ge.x + M[1] * ge.y + M[2] * ge.z, M[4] * ge.x + M[5] * ge.y + M[6] * ge.z,
+                                  M[8] * ge.x + M[9] * ge.y + M[10] * ge.z);
+                gP = gPd;
+                if (prm.displacement) {
+                    const float nd = vdot(nh[k], gPd);
+                    gamp += nzv[k] * nd;
+                    gP = gP + gn[k] * (ob.disp_amplitude * nd);
+                    if (nlen[k] > 0.0f) {
+                        const V3 gnh = gPd * dd[k];
+                        const V3 gN = (gnh - nh[k] * vdot(nh[k], gnh)) * (1.0f / nlen[k]);
+                        gPv = vcross(Pu[k], gN);
+                        gPu = vcross(gN, Pv[k]);
+                    }
                 }
             }
-            const int nu = g % G1, nv = g / G1;
-            const float uu = rec.umin + (rec.umax - rec.umin) * ((float)nu / (float)N);
-            const float vv = rec.vmin + (rec.vmax - rec.vmin) * ((float)nv / (float)N);
-            float su = 1.0f - uu, sv = 1.0f - vv, a[4], b[4], da[4], db[4];
-            bern_u(uu, su, a); bern_v(vv, sv, b); dbern(uu, su, da); dbern(vv, sv, db);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int c = 4 * i + j;
-                    const float w = a[i] * b[j], wu = da[i] * b[j], wv = a[i] * db[j];
-                    gcp[3 * c + 0] += w * gP.x + wu * gPu.x + wv * gPv.x;
-                    gcp[3 * c + 1] += w * gP.y + wu * gPu.y + wv * gPv.y;
-                    gcp[3 * c + 2] += w * gP.z + wu * gPu.z + wv * gPv.z;
-                }
-        }
-        // reduce over the work-group: the 52 sums of each wave at once (lane
-        // i ends with sum i), then across waves
-        {
-            float red[64];
-#pragma unroll
-            for (int i = 0; i < 48; ++i) red[i] = gcp[i];
-            red[48] = gamp; red[49] = ga0; red[50] = ga1; red[51] = ga2;
-#pragma unroll
-            for (int i = 52; i < 64; ++i) red[i] = 0.0f;
-            const float t = wave_sums64(red);
-            if (lane < 52) s_red[wave][lane] = t;
+            s_gp[g][0] = gP.x;  s_gp[g][1] = gP.y;  s_gp[g][2] = gP.z;
+            s_gp[g][3] = gPu.x; s_gp[g][4] = gPu.y; s_gp[g][5] = gPu.z;
+            s_gp[g][6] = gPv.x; s_gp[g][7] = gPv.y; s_gp[g][8] = gPv.z;
         }
         __syncthreads();
-        if (tid < 52) {
+
+        // control points, separably over the grid's rows and columns:
+        //   dL/dP_ij = sum_v b_j(v) [sum_u a_i(u) gP + a_i'(u) gPu] + b_j'(v) [sum_u a_i(u) gPv]
+        // rows first: s_row[nv][i][q], q < 3 the bracket, q >= 3 the gPv sum
+        for (int o = tid; o < G1 * 24; o += Q) {
+            const int nv = o / 24, rem = o % 24, i = rem / 6, q = rem % 6;
+            const float* gp = &s_gp[nv * G1][0];
             float t = 0.0f;
-            for (int w = 0; w < W; ++w) t += s_red[w][tid];
+            if (q < 3) {
+                for (int nu = 0; nu < G1; ++nu)
+                    t += s_wu[nu][i] * gp[nu * 9 + q] + s_wu[nu][4 + i] * gp[nu * 9 + 3 + q];
+            } else {
+                for (int nu = 0; nu < G1; ++nu) t += s_wu[nu][i] * gp[nu * 9 + 3 + q];
+            }
+            s_row[nv][i][q] = t;
+        }
+        // amplitude and albedo sums of each wave, four chains side by side
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            gamp += __shfl_xor(gamp, o);
+            ga0 += __shfl_xor(ga0, o);
+            ga1 += __shfl_xor(ga1, o);
+            ga2 += __shfl_xor(ga2, o);
+        }
+        if (lane == 0) { s_red[wave][0] = gamp; s_red[wave][1] = ga0; s_red[wave][2] = ga1; s_red[wave][3] = ga2; }
+        __syncthreads();
+        if (tid < 48) {
+            const int c = tid / 3, comp = tid % 3, i = c >> 2, j = c & 3;
+            float t = 0.0f;
+            for (int nv = 0; nv < G1; ++nv)
+                t += s_wv[nv][j] * s_row[nv][i][comp] + s_wv[nv][4 + j] * s_row[nv][i][3 + comp];
+            if (t != 0.0f) atomicAdd(&grad_cps[(size_t)ob.grad_offset + (size_t)rec.pid * 48 + tid], t);
+        } else if (tid < 52) {
+            float t = 0.0f;
+            for (int w = 0; w < W; ++w) t += s_red[w][tid - 48];
             if (t != 0.0f) {
-                if (tid < 48) atomicAdd(&grad_cps[(size_t)ob.grad_offset + (size_t)rec.pid * 48 + tid], t);
-                else if (tid == 48) atomicAdd(&grad_amp[ob.draw_index], t);
+                if (tid == 48) atomicAdd(&grad_amp[ob.draw_index], t);
                 else atomicAdd(&grad_color[ob.draw_index * 4 + (tid - 49)], t);
             }
         }
