@@ -136,6 +136,7 @@ struct mp_context {
     int split_spread = 2;        // persistent split: 0 three samples per lane; 1 one sample
                                  // per lane on narrow steps; 2 also two levels on the narrowest
     int split_budget = 8;        // LOCAL persistent split: steps before rebalancing (0: none)
+    int fused_order = 1;         // canonical order: count scan inside the scatter launch
     mp_stats stats;
     std::vector<int32_t> pass_counts;
     bool frame_valid = false;
@@ -415,6 +416,7 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     if (const char* e = getenv("MP_STACK_CAP")) ctx->stack_cap = std::max(256, atoi(e));
     if (const char* e = getenv("MP_SPLIT_SPREAD")) ctx->split_spread = atoi(e);
     if (const char* e = getenv("MP_SPLIT_BUDGET")) ctx->split_budget = std::max(0, atoi(e));
+    if (const char* e = getenv("MP_FUSED_ORDER")) ctx->fused_order = atoi(e) != 0;
     if (const char* e = getenv("MP_SPLIT_WAVES")) ctx->split_waves = atoi(e) == 3 ? 3 : 2;
     if (const char* e = getenv("MP_SPLIT_ENGINE")) ctx->split_engine = atoi(e);
     if (const char* e = getenv("MP_CPU_THREADS")) ctx->cpu_threads = atoi(e);
@@ -864,7 +866,7 @@ static int launch_back_end(mp_context* ctx, hipEvent_t* ev = nullptr)
 {
     CanonBuffers cb = canon_buffers(ctx);
     HIPCHK(ctx, launch_canonicalize(ctx->d_objs, ctx->d_views, ctx->n_views, cb, ctx->d_fc,
-                                    ctx->num_cus, ctx->stream));
+                                    ctx->num_cus, ctx->fused_order, ctx->stream));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
     if (!ctx->cfg.dummy_render)
         HIPCHK(ctx, launch_raster(ctx->cfg.reyes_patch_size, ctx->d_canon, ctx->d_objs, ctx->d_views,

@@ -88,7 +88,7 @@ hipError_t launch_move_segment(const DevObject* objs, const RangeRec* pad, uint3
 uint32_t segment_blocks(uint32_t n);
 
 hipError_t launch_canonicalize(const DevObject* objs, DevView* views, int n_views,
-                               const CanonBuffers& cb, FrameCounters* fc, int num_cus,
+                               const CanonBuffers& cb, FrameCounters* fc, int num_cus, int fused,
                                hipStream_t s);
 hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject* objs,
                          const DevView* views, const DevParams& prm, FrameCounters* fc,

@@ -17,6 +17,7 @@
 //                                  kernels/reyes.cl); 64-bit depth|id atomicMin
 //  resolve    k_resolve            key buffer -> RGBA8 / depth / ids, and clear
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <type_traits>
 
 #include "mp_device.h"
@@ -1094,6 +1095,78 @@ __global__ __launch_bounds__(256) void k_scatter(const DevObject* __restrict__ o
     }
 }
 
+// Frames of up to kFusedScan patches: every work-group scans the per-patch
+// counts itself (four per thread, offsets kept in LDS) and scatters its share
+// of the draw list from there, as k_scatter does: one launch instead of a scan
+// launch and a scatter launch.  It relies on the counts being those of this
+// frame only and on zero cursors: k_rank clears both after every scatter.
+constexpr uint32_t kFusedScan = 4096;
+
+__global__ __launch_bounds__(1024) void k_scan_scatter(const DevObject* __restrict__ objs, CanonBuffers cb,
+                                                       FrameCounters* fc, DevView* views, int n_views)
+{
+    __shared__ uint32_t s_off[kFusedScan + 1];
+    __shared__ uint32_t lw[32];
+    const uint32_t n = cb.n_gpids;
+    const uint32_t t = threadIdx.x;
+    uint32_t c[4], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t i = t * 4u + k;
+        c[k] = i < n ? cb.counts[i] : 0u;
+        sum += c[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan_1024(sum, lw, total);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t i = t * 4u + k;
+        if (i < n) s_off[i] = run;
+        run += c[k];
+    }
+    if (t == 0) s_off[n] = total;
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        if (t == 0) fc->total_ranges = total;
+        for (int v = (int)t; v < n_views; v += 1024) {
+            const uint32_t b = s_off[views[v].first_gpid];
+            views[v].range_base = b;
+            views[v].range_count = s_off[views[v].gpid_end] - b;
+        }
+    }
+    const int lane = t & 63;
+    const uint32_t stride = gridDim.x * 1024u;
+    const uint32_t lim = (total + 1023u) & ~1023u;
+    for (uint32_t i = blockIdx.x * 1024u + t; i < lim; i += stride) {
+        const bool valid = i < total;
+        RangeRec r;
+        uint32_t g = 0;
+        if (valid) {
+            r = cb.draw[i];
+            g = gpid_of(objs, r);
+        }
+        uint64_t m = __ballot(valid);
+        int my_leader = 0;
+        uint32_t my_rank = 0, lead_cnt = 0;
+        while (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            const uint32_t lg = __shfl(g, leader);
+            const uint64_t same = __ballot(valid && g == lg) & m;
+            if ((same >> lane) & 1ull) { my_leader = leader; my_rank = lanes_below(same); }
+            if (lane == leader) lead_cnt = (uint32_t)__popcll(same);
+            m &= ~same;
+        }
+        uint32_t b = 0;
+        if (lead_cnt) b = s_off[g] + atomicAdd(&cb.cursor[g], lead_cnt);
+        const uint32_t slot = (uint32_t)__shfl((int)b, my_leader) + my_rank;
+        if (valid) {
+            cb.bucket[slot] = r;
+            cb.keys[slot] = canonical_key(r);
+            cb.span[slot] = make_uint2(s_off[g], s_off[g + 1]);
+        }
+    }
+}
+
 // Rank of each bucketed range among its patch's ranges (its canonical
 // index = the patch's first slot + the number of its ranges with a smaller
 // key).  A work-group takes 256 consecutive bucketed ranges; the keys of
@@ -1107,6 +1180,12 @@ __global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, const FrameCounte
     __shared__ uint32_t s_lo[4], s_hi[4];
     const uint32_t total = fc->total_ranges;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    // every path leaves the per-patch counts and bucket cursors zero for the
+    // next frame (the scatter, fused or not, has used them)
+    for (uint32_t i = blockIdx.x * 256u + tid; i < cb.n_gpids; i += gridDim.x * 256u) {
+        cb.counts[i] = 0;
+        cb.cursor[i] = 0;
+    }
     for (uint32_t j0 = blockIdx.x * 256u; j0 < total; j0 += gridDim.x * 256u) {
         const uint32_t j = j0 + tid;
         const bool valid = j < total;
@@ -1156,10 +1235,18 @@ __global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, const FrameCounte
 }
 
 hipError_t launch_canonicalize(const DevObject* objs, DevView* views, int n_views,
-                               const CanonBuffers& cb, FrameCounters* fc, int num_cus,
+                               const CanonBuffers& cb, FrameCounters* fc, int num_cus, int fused,
                                hipStream_t s)
 {
     hipError_t e;
+    if (fused && cb.n_gpids <= kFusedScan) {
+        const uint32_t want = (cb.cap + 1023u) / 1024u;
+        const int grid = (int)std::max(1u, std::min<uint32_t>((uint32_t)num_cus, want));
+        hipLaunchKernelGGL(k_scan_scatter, dim3(grid), dim3(1024), 0, s, objs, cb, fc, views, n_views);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rank, dim3(num_cus * 4), dim3(256), 0, s, cb, fc);
+        return hipGetLastError();
+    }
     if (cb.n_gpids <= 2u * kScanBlock) {
         hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, cb, views, n_views, fc);
     } else {
