@@ -1234,6 +1234,14 @@ __global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, const FrameCounte
     }
 }
 
+// rank work-groups: one per 256 list entries the lists can hold, at most four
+// per CU (each stages up to 32 KB of keys)
+static int rank_grid(const CanonBuffers& cb, int num_cus)
+{
+    const uint32_t want = std::max(1u, (cb.cap + 255u) / 256u);
+    return (int)std::min<uint32_t>(want, (uint32_t)num_cus * 4u);
+}
+
 hipError_t launch_canonicalize(const DevObject* objs, DevView* views, int n_views,
                                const CanonBuffers& cb, FrameCounters* fc, int num_cus, int fused,
                                hipStream_t s)
@@ -1244,7 +1252,7 @@ hipError_t launch_canonicalize(const DevObject* objs, DevView* views, int n_view
         const int grid = (int)std::max(1u, std::min<uint32_t>((uint32_t)num_cus, want));
         hipLaunchKernelGGL(k_scan_scatter, dim3(grid), dim3(1024), 0, s, objs, cb, fc, views, n_views);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_rank, dim3(num_cus * 4), dim3(256), 0, s, cb, fc);
+        hipLaunchKernelGGL(k_rank, dim3(rank_grid(cb, num_cus)), dim3(256), 0, s, cb, fc);
         return hipGetLastError();
     }
     if (cb.n_gpids <= 2u * kScanBlock) {
@@ -1263,7 +1271,7 @@ hipError_t launch_canonicalize(const DevObject* objs, DevView* views, int n_view
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, objs, cb, fc, views, n_views);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_rank, dim3(num_cus * 4), dim3(256), 0, s, cb, fc);
+    hipLaunchKernelGGL(k_rank, dim3(rank_grid(cb, num_cus)), dim3(256), 0, s, cb, fc);
     return hipGetLastError();
 }
 
