@@ -1,12 +1,13 @@
 """GPU parity on edge cases against the oracle: an empty frame, a patch
 through the camera plane (near-plane splits, culls behind the eye),
 degenerate patches (a point, a line), ragged window sizes (partial 8x8
-tiles), mixed Bezier / Gregory objects with a patch set drawn twice, and the
-split depth limit."""
+tiles), mixed Bezier / Gregory objects with a patch set drawn twice, the
+split depth limit, and one renderer reused across frames of different patch
+counts."""
 import numpy as np
 import pytest
 
-from micropolis_amd import scenes
+from micropolis_amd import Renderer, scenes
 from micropolis_amd.renderer import Projection
 from gpu_util import assert_frame_equal, render_plain
 
@@ -132,4 +133,36 @@ def test_atomic_rate_measurement():
     r, got = render_plain(w, displacement=0)
     rate = r.measure_atomic_rate(1 << 20)
     assert 1e8 < rate < 1e14
+    r.close()
+
+
+def test_one_renderer_across_patch_counts(oracle):
+    """Frames of more and of at most 4096 patches on one renderer, in both
+    orders: the canonical order's count scan runs in its own launch or inside
+    the scatter launch, and each frame must start from zero per-patch counts
+    and bucket cursors whichever ran before."""
+    rng = np.random.default_rng(11)
+    opts = dict(window_size=(320, 240), bound_n_split_method="LOCAL", bound_n_split_limit=8.0,
+                reyes_patch_size=8, backface_culling=1)
+    frames = []
+    for k, n in enumerate((6000, 1500, 5000, 3000)):
+        radius = 2.0 * n ** (1.0 / 3.0)
+        pats = scenes.random_patches(rng, n, radius)
+        eye = (radius * 2.2, 0.3 * radius, radius * 0.5)
+        o = scenes._obj(10 + k, pats, scenes.BEZIER, scenes.look_at(eye, (0, 0, 0)), (0.8, 0.6, 0.3, 1.0))
+        frames.append(scenes.Workload("reuse%d" % k, opts, [scenes.View(Projection(60.0, 0.01, (320, 240)), [o])]))
+    r = Renderer(**opts)
+    for w in frames:
+        print("case", w.name, flush=True)
+        v = w.views[0]
+        o = v.objects[0]
+        r.load_patches(o.handle, o.patches, o.patch_type)
+        r.prepare()
+        r.draw_patches(o.handle, o.matrix, v.projection, o.color)
+        r.finish()
+        got = r.read_framebuffer()
+        got["ranges"] = r.ranges()
+        ref = oracle.render(w)[0]
+        assert_frame_equal(got, ref, w.name)
+        r.unload_patches(o.handle)
     r.close()
