@@ -2017,7 +2017,10 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
     __shared__ int2 s_pxl[G];
     __shared__ float s_ge[G][3];
     __shared__ float s_red[W][4];
-    __shared__ float s_gp[G][9];          // dL/d(P, dP/du, dP/dv) per grid point
+    // per grid point: the forward state the reverse pass needs (dP/du,
+    // dP/dv, unit normal, noise gradient, |normal|, noise, displacement),
+    // then, in the same row, dL/d(P, dP/du, dP/dv)
+    __shared__ float s_gp[G][16];
     __shared__ float s_row[G1][4][6];     // per grid row and i: the u sums
     __shared__ float s_wu[G1][8];         // a_i(u), a_i'(u) per grid column
     __shared__ float s_wv[G1][8];         // b_j(v), b_j'(v) per grid row
@@ -2052,15 +2055,16 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
         }
         __syncthreads();
 
-        // forward state of the grid points owned by this thread
-        V3 Pu[PPT], Pv[PPT], nh[PPT], gn[PPT];
-        float nlen[PPT], nzv[PPT], dd[PPT];
+        // forward state of the grid points owned by this thread (to LDS)
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             // every lane of a wave that owns any point runs (lanes past the
             // last point repeat it and store nothing): the noise lattice
             // comes from the lane window
             const int g = tid + k * Q;
+            // the control points are read again from LDS for each point
+            // instead of being held in 64 registers across the loop
+            asm volatile("" ::: "memory");
             if ((tid & ~63) + k * Q < G) {
                 const int gc = min(g, G - 1);
                 const int nu = gc % G1, nv = gc / G1;
@@ -2089,7 +2093,12 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
                     nzz = noise3_grad<true>(prm, P[0], P[1], P[2], gz);
                     d = ob.disp_amplitude * nzz;
                 }
-                Pu[k] = pu; Pv[k] = pv; nh[k] = h; gn[k] = gz; nlen[k] = len; nzv[k] = nzz; dd[k] = d;
+                if (g < G) {
+                    float* st = s_gp[g];
+                    st[0] = pu.x; st[1] = pu.y; st[2] = pu.z; st[3] = pv.x; st[4] = pv.y; st[5] = pv.z;
+                    st[6] = h.x; st[7] = h.y; st[8] = h.z; st[9] = gz.x; st[10] = gz.y; st[11] = gz.z;
+                    st[12] = len; st[13] = nzz; st[14] = d;
+                }
                 // forward displaced eye position and pixel position: P + d * unit
                 // normal with the same operations as the forward dice
                 float Pd[4] = { P[0], P[1], P[2], P[3] };
@@ -2175,6 +2184,10 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
             const int g = tid + k * Q;
             if (g >= G) continue;
             const V3 ge = mk(s_ge[g][0], s_ge[g][1], s_ge[g][2]);
+            float* st = s_gp[g];
+            const V3 Pu = mk(st[0], st[1], st[2]), Pv = mk(st[3], st[4], st[5]);
+            const V3 nh = mk(st[6], st[7], st[8]), gn = mk(st[9], st[10], st[11]);
+            const float nlen = st[12], nzv = st[13], dd = st[14];
             V3 gP = mk(0.f, 0.f, 0.f), gPu = mk(0.f, 0.f, 0.f), gPv = mk(0.f, 0.f, 0.f);
             if (ge.x != 0.0f || ge.y != 0.0f || ge.z != 0.0f) {
                 const float* M = ob.mv;
@@ -2182,20 +2195,20 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
                                   M[8] * ge.x + M[9] * ge.y + M[10] * ge.z);
                 gP = gPd;
                 if (prm.displacement) {
-                    const float nd = vdot(nh[k], gPd);
-                    gamp += nzv[k] * nd;
-                    gP = gP + gn[k] * (ob.disp_amplitude * nd);
-                    if (nlen[k] > 0.0f) {
-                        const V3 gnh = gPd * dd[k];
-                        const V3 gN = (gnh - nh[k] * vdot(nh[k], gnh)) * (1.0f / nlen[k]);
-                        gPv = vcross(Pu[k], gN);
-                        gPu = vcross(gN, Pv[k]);
+                    const float nd = vdot(nh, gPd);
+                    gamp += nzv * nd;
+                    gP = gP + gn * (ob.disp_amplitude * nd);
+                    if (nlen > 0.0f) {
+                        const V3 gnh = gPd * dd;
+                        const V3 gN = (gnh - nh * vdot(nh, gnh)) * (1.0f / nlen);
+                        gPv = vcross(Pu, gN);
+                        gPu = vcross(gN, Pv);
                     }
                 }
             }
-            s_gp[g][0] = gP.x;  s_gp[g][1] = gP.y;  s_gp[g][2] = gP.z;
-            s_gp[g][3] = gPu.x; s_gp[g][4] = gPu.y; s_gp[g][5] = gPu.z;
-            s_gp[g][6] = gPv.x; s_gp[g][7] = gPv.y; s_gp[g][8] = gPv.z;
+            st[0] = gP.x;  st[1] = gP.y;  st[2] = gP.z;
+            st[3] = gPu.x; st[4] = gPu.y; st[5] = gPu.z;
+            st[6] = gPv.x; st[7] = gPv.y; st[8] = gPv.z;
         }
         __syncthreads();
 
@@ -2208,9 +2221,9 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
             float t = 0.0f;
             if (q < 3) {
                 for (int nu = 0; nu < G1; ++nu)
-                    t += s_wu[nu][i] * gp[nu * 9 + q] + s_wu[nu][4 + i] * gp[nu * 9 + 3 + q];
+                    t += s_wu[nu][i] * gp[nu * 16 + q] + s_wu[nu][4 + i] * gp[nu * 16 + 3 + q];
             } else {
-                for (int nu = 0; nu < G1; ++nu) t += s_wu[nu][i] * gp[nu * 9 + 3 + q];
+                for (int nu = 0; nu < G1; ++nu) t += s_wu[nu][i] * gp[nu * 16 + 3 + q];
             }
             s_row[nv][i][q] = t;
         }
