@@ -99,3 +99,31 @@ def test_work_sharing_off_gives_the_same_frame(monkeypatch):
     r2, b = render_plain(w)
     r2.close()
     assert np.array_equal(a["ids"], b["ids"]) and np.array_equal(a["depth"], b["depth"])
+
+
+def test_small_batch_views_match_oracle(oracle):
+    """Three views of under 4096 patches in all (the canonical order's count
+    scan then runs inside the scatter launch, which also sets each view's
+    range base): ranges per view and images against the oracle."""
+    rng = np.random.default_rng(21)
+    opts = dict(window_size=(320, 240), bound_n_split_method="LOCAL", bound_n_split_limit=8.0,
+                reyes_patch_size=8, backface_culling=1)
+    views = []
+    for k, n in enumerate((900, 400, 1300)):
+        radius = 2.0 * n ** (1.0 / 3.0)
+        pats = scenes.random_patches(rng, n, radius)
+        eye = (radius * 2.0, 0.4 * radius, -radius * 0.8)
+        o = scenes._obj(50 + k, pats, scenes.BEZIER, scenes.look_at(eye, (0, 0, 0)), (0.5, 0.7, 0.9, 1.0))
+        views.append(scenes.View(scenes.Projection(60.0, 0.01, (320, 240)), [o]))
+    w = scenes.Workload("small_batch", opts, views)
+    rb = render_batch(w)
+    allr = rb.ranges()
+    off = 0
+    for vi, v in enumerate(w.views):
+        ref = oracle.render(scenes.Workload("v", opts, [v]))[0]
+        got = rb.read_framebuffer(view=vi)
+        got["ranges"] = allr[off:off + len(ref["ranges"])]
+        off += len(ref["ranges"])
+        assert_frame_equal(got, ref, "small batch view %d" % vi)
+    assert off == len(allr)
+    rb.close()
