@@ -265,6 +265,94 @@ MP_DEV float vnoise(float x, float y, float z, uint32_t seed, int scheme = kNois
     return lerpf_(y0, y1, wz);
 }
 
+// Lane windows kept by one wave across calls (the raster's dice passes and
+// the ranges after them, which mostly fall in the same lattice cells of the
+// low octaves): per octave the window's base corner and this lane's hashed
+// corner value.  A call whose cells all fit the kept window reads it with no
+// hashing; otherwise a new window around lane 32's cell is hashed and kept
+// (as window_corners does), and if even that does not fit, each lane hashes
+// its own eight corners.  The values are lattice()'s either way.  Whole
+// waves only.  Two octaves: keeping four costs the raster spilled registers.
+constexpr int kKeptOctaves = 2;
+struct NoiseWindows {
+    uint32_t bx[kKeptOctaves], by[kKeptOctaves], bz[kKeptOctaves];
+    float hv[kKeptOctaves];
+    bool valid[kKeptOctaves];
+    MP_DEV void reset()
+    {
+#pragma unroll
+        for (int k = 0; k < kKeptOctaves; ++k) {
+            bx[k] = by[k] = bz[k] = 0u;
+            hv[k] = 0.0f;
+            valid[k] = false;
+        }
+    }
+};
+
+MP_DEV float vnoise_kept(float x, float y, float z, uint32_t seed, uint32_t& bx, uint32_t& by, uint32_t& bz,
+                         float& hv, bool& valid)
+{
+    float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
+    float wx = fade(x - fx), wy = fade(y - fy), wz = fade(z - fz);
+    uint32_t dx = (uint32_t)ix - bx, dy = (uint32_t)iy - by, dz = (uint32_t)iz - bz;
+    bool fits = valid && __all(dx <= 2u && dy <= 2u && dz <= 2u);
+    if (!fits) {
+        const uint32_t nx = (uint32_t)__builtin_amdgcn_readlane(ix, 32) - 1u;
+        const uint32_t ny = (uint32_t)__builtin_amdgcn_readlane(iy, 32) - 1u;
+        const uint32_t nz = (uint32_t)__builtin_amdgcn_readlane(iz, 32) - 1u;
+        dx = (uint32_t)ix - nx; dy = (uint32_t)iy - ny; dz = (uint32_t)iz - nz;
+        if (__all(dx <= 2u && dy <= 2u && dz <= 2u)) {
+            const uint32_t k = threadIdx.x & 63u;
+            bx = nx; by = ny; bz = nz;
+            hv = lattice((int32_t)(nx + (k & 3u)), (int32_t)(ny + ((k >> 2) & 3u)), (int32_t)(nz + (k >> 4)), seed);
+            valid = true;
+            fits = true;
+        }
+    }
+    float c000, c100, c010, c110, c001, c101, c011, c111;
+    if (fits) {
+        const int a = (int)((dx + 4u * dy + 16u * dz) << 2);
+        c000 = lane_value(a, hv);      c100 = lane_value(a + 4, hv);
+        c010 = lane_value(a + 16, hv); c110 = lane_value(a + 20, hv);
+        c001 = lane_value(a + 64, hv); c101 = lane_value(a + 68, hv);
+        c011 = lane_value(a + 80, hv); c111 = lane_value(a + 84, hv);
+    } else {
+        c000 = lattice(ix, iy, iz, seed);         c100 = lattice(ix + 1, iy, iz, seed);
+        c010 = lattice(ix, iy + 1, iz, seed);     c110 = lattice(ix + 1, iy + 1, iz, seed);
+        c001 = lattice(ix, iy, iz + 1, seed);     c101 = lattice(ix + 1, iy, iz + 1, seed);
+        c011 = lattice(ix, iy + 1, iz + 1, seed); c111 = lattice(ix + 1, iy + 1, iz + 1, seed);
+    }
+    float x00 = lerpf_(c000, c100, wx), x10 = lerpf_(c010, c110, wx);
+    float x01 = lerpf_(c001, c101, wx), x11 = lerpf_(c011, c111, wx);
+    float y0 = lerpf_(x00, x10, wy), y1 = lerpf_(x01, x11, wy);
+    return lerpf_(y0, y1, wz);
+}
+
+// noise3 of a whole wave with kept windows for the first kKeptOctaves
+// octaves and the lane window for the rest (the same value as noise3).
+MP_DEV float noise3_kept(const DevParams& prm, float x, float y, float z, NoiseWindows& w)
+{
+    float acc = 0.0f, amp = 1.0f, norm = 0.0f, freq = prm.disp_frequency;
+#pragma unroll
+    for (int k = 0; k < kKeptOctaves; ++k) {
+        if (k < prm.disp_octaves) {
+            acc = acc + amp * vnoise_kept(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, w.bx[k],
+                                          w.by[k], w.bz[k], w.hv[k], w.valid[k]);
+            norm = norm + amp;
+            amp = amp * 0.5f;
+            freq = freq * 2.0f;
+        }
+    }
+    for (int k = kKeptOctaves; k < prm.disp_octaves; ++k) {
+        acc = acc + amp * vnoise(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, kNoiseLaneWindow);
+        norm = norm + amp;
+        amp = amp * 0.5f;
+        freq = freq * 2.0f;
+    }
+    return acc / norm;
+}
+
 // full_wave: every lane of the wave runs this call (the lane window needs
 // all 64 lanes); otherwise the window scheme falls back to the scalar one.
 MP_DEV float noise3(const DevParams& prm, float x, float y, float z, bool full_wave = false)
@@ -387,14 +475,16 @@ MP_DEV void displaced_point(const float4* __restrict__ cp, int type, float u, fl
 
 // Same with the normal's column sums precomputed (col: 8 float4 of the
 // point's u column).
+// kept: lattice windows the calling wave keeps across calls (whole waves).
 MP_DEV void displaced_point_cols(const float4* __restrict__ cp, const float4* __restrict__ col,
                                  float u, float v, const DevParams& prm, float amp, float out[4],
-                                 bool full_wave = false)
+                                 bool full_wave = false, NoiseWindows* kept = nullptr)
 {
     eval_bezier(cp, u, v, out);
     float h[3];
     normal_from_columns(col, v, h);
-    float d = amp * noise3(prm, out[0], out[1], out[2], full_wave);
+    float d = amp * (kept ? noise3_kept(prm, out[0], out[1], out[2], *kept)
+                          : noise3(prm, out[0], out[1], out[2], full_wave));
     out[0] = out[0] + d * h[0];
     out[1] = out[1] + d * h[1];
     out[2] = out[2] + d * h[2];

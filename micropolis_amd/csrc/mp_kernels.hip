@@ -1529,6 +1529,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __syncthreads();
     int buf = 0;
     float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    NoiseWindows nw;   // lattice windows of the low octaves, kept across passes and ranges
+    nw.reset();
     while (r < total) {
         // next range (the following chunk is claimed one range ahead) and its
         // control points: loaded now, stored to the other LDS buffer after
@@ -1571,13 +1573,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             // noise exchanges values across the wave); lanes past the last
             // point repeat it and store nothing
             for (int gb = wave * 64; gb < G; gb += NT) {
+                // control points re-read from LDS each pass (registers for the
+                // kept lattice windows)
+                asm volatile("" ::: "memory");
                 const int g = min(gb + lane, G - 1);
                 const int nu = g % G1, nv = g / G1;
                 const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
                 const float uu = rec.umin + (rec.umax - rec.umin) * tu;
                 const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
                 float P[4], e[4], q[4];
-                if (disp_cols) displaced_point_cols(s_cp, s_col + 8 * nu, uu, vv, prm, ob.disp_amplitude, P, true);
+                if (disp_cols) displaced_point_cols(s_cp, s_col + 8 * nu, uu, vv, prm, ob.disp_amplitude, P, true, &nw);
                 else if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
                 else eval_patch(s_cp, ob.type, uu, vv, P);
                 mat4_mul(ob.mv, P, e);
