@@ -20,8 +20,18 @@
  *                      reference raster written independently in the test
  *   or_prefix_sum      the reference's own scan self-test (main.cpp:343-400,
  *                      srand(43), rand()%8+1) with glibc's rand sequence
- *   or_noise           lattice/interpolation invariants (values at lattice
- *                      points, continuity, octave sum bound)
+ *   or_noise           the seeded lattice value at integer points, smoothstep
+ *                      fade with flat ends, trilinear corner mean at cell
+ *                      centres, octave halving / normalisation / reseeding
+ *   quad_lambert       planar parallelograms of known orientation:
+ *   (or_shade's term)  max(n . normalize(4,3,8), 0) from the FP64 plane
+ *                      normal, back-lit clamp, NaN for a zero corner sum
+ *   or_resolve         FP64 0.015*dc + s*dc colour, clear (rgb^2.2, 1000),
+ *                      depth decode and ids of a hand-built key buffer
+ *   or_make_view       glm::perspective's published entries, frustum planes
+ *                      at NDC +-1, fx = fy = H / (2 tan(fovy/2))
+ *   or_to_srgb8        FP64 pow(c, 0.454545) * 255 + 0.5 off byte boundaries
+ *   or_framebuffer_*   tiled bijection of the padded window
  *   or_bound_cpu       FP64 de Casteljau rows vs the glm-order evaluation,
  *   or_eval_patch_cpu  analytic sizes of planar patches through
  *   or_projection_bound Projection::bound, hand-built cull/drop cases, the
