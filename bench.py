@@ -99,8 +99,10 @@ def measure_adjoint(steps):
     g_amp = torch.empty((V,), device="cuda")
     torch.cuda.synchronize()
     bwd = []
+    serial = r.frame_serial()
     for _ in range(steps + 1):
-        r.backward(dl.data_ptr(), g_cps.data_ptr(), g_col.data_ptr(), g_amp.data_ptr())
+        r.backward(serial, dl.data_ptr(), dl.numel(), g_cps.data_ptr(), g_cps.numel(), g_col.data_ptr(),
+                   g_amp.data_ptr(), V)
         bwd.append(r.stats()["backward_ms"])
     bwd_ms = float(np.median(bwd[1:]))
     r.close()

@@ -128,7 +128,12 @@ int mp_unload_patches(mp_context* ctx, uint64_t handle);
 int mp_prepare(mp_context* ctx);
 
 /* Renderer::draw_patches (src/Reyes/RendererCL.cpp:170-200).  mv is the
- * column-major modelview (Scene.cpp:190), color the diffuse colour. */
+ * column-major modelview (Scene.cpp:190), color the diffuse colour.
+ * Every draw of one frame renders into the window with one projection: the
+ * viewport must equal the window size and a later draw's fovy / near plane
+ * must equal the first draw's (MP_E_INVALID otherwise; the reference could
+ * change the projection between draws, RendererCL.cpp:172-178, which no
+ * caller of it does: Scene::draw passes one projection per frame). */
 int mp_draw_patches(mp_context* ctx, uint64_t handle, const float mv[16],
                     const mp_projection* projection, const float color[4]);
 
@@ -288,12 +293,30 @@ int mp_measure_atomic_rate(mp_context* ctx, uint32_t n, double* atomics_per_seco
  *   grad_color     : per object 4 floats (albedo gradient, alpha 0)
  *   grad_amplitude : per object 1 float
  * Outputs are overwritten.  Bezier objects only.  Blocks until done. */
-int mp_backward(mp_context* ctx, const float* dl_dcolor, float* grad_cps, float* grad_color,
-                float* grad_amplitude);
+int mp_backward(mp_context* ctx, uint64_t frame_serial, const float* dl_dcolor, size_t dl_floats,
+                float* grad_cps, size_t grad_cps_floats, float* grad_color, float* grad_amplitude,
+                int32_t n_objects);
+/*   frame_serial    : mp_frame_serial() of the frame whose visibility and
+ *                     control points the gradients are taken for; MP_E_STATE
+ *                     if the context has rendered another frame (or its
+ *                     patches changed) since
+ *   dl_floats       : floats in dl_dcolor, must equal 4 * the pixels of all
+ *                     views; grad_cps_floats: floats in grad_cps, must equal
+ *                     the objects' n_patches*16*3 summed; n_objects: entries
+ *                     of grad_color (x4) and grad_amplitude, must equal the
+ *                     draws of the frame (MP_E_INVALID otherwise) */
+
+/* Serial number of the last finished frame or batch (0: none).  It changes
+ * with every mp_finish / mp_batch_finish and with every change of a loaded
+ * patch set (mp_load_patches, mp_unload_patches, mp_update_patches_device);
+ * replays of a frame keep it. */
+int mp_frame_serial(mp_context* ctx, uint64_t* serial);
 
 /* Device-to-device readback of a view's linear RGBA32F colour and ids
- * (either may be NULL; needs write_rgba32 for the colour). */
-int mp_batch_read_device(mp_context* ctx, int32_t view, float* rgba32, uint32_t* ids);
+ * (either may be NULL; needs write_rgba32 for the colour).  n_pixels is the
+ * caller's buffer size in pixels (rgba32 holds 4 floats per pixel) and must
+ * equal the view's width * height (MP_E_INVALID otherwise). */
+int mp_batch_read_device(mp_context* ctx, int32_t view, float* rgba32, uint32_t* ids, size_t n_pixels);
 
 /* Replace a loaded patch set's control points from device memory (xyz
  * floats, same count as loaded). */

@@ -101,8 +101,9 @@ EXPORTS = [
     ("mp_replay_stages", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
     ("mp_measure_atomic_rate", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double)]),
     ("mp_display_thresholds", C.c_int, [C.POINTER(C.c_float)]),
-    ("mp_backward", C.c_int, [_P, _P, _P, _P, _P]),
-    ("mp_batch_read_device", C.c_int, [_P, C.c_int32, _P, _P]),
+    ("mp_backward", C.c_int, [_P, C.c_uint64, _P, C.c_size_t, _P, C.c_size_t, _P, _P, C.c_int32]),
+    ("mp_frame_serial", C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    ("mp_batch_read_device", C.c_int, [_P, C.c_int32, _P, _P, C.c_size_t]),
     ("mp_update_patches_device", C.c_int, [_P, C.c_uint64, _P, C.c_size_t]),
     ("mp_dump_trace", C.c_int, [_P, C.c_char_p]),
     ("mp_dump_stats", C.c_int, [_P, C.c_char_p]),

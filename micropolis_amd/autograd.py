@@ -7,10 +7,15 @@ hand-written reverse pass (mp_backward): gradients with respect to the
 control points, the displacement amplitude and the albedo, with visibility
 held fixed to the forward visible-id buffers (silhouette terms ignored) and
 micropolygon grids recomputed per range.
+
+Every view of one call has the same viewport (the outputs are one stacked
+tensor).  The backward is taken for the frame its forward rendered: if the
+renderer renders another frame (or its patches change) in between, the
+library refuses (MicropolisError) instead of differentiating the wrong frame.
 """
 import torch
 
-from .renderer import BEZIER
+from .renderer import BEZIER, MicropolisError
 
 _HANDLE = 0x5EED
 
@@ -37,8 +42,9 @@ class _RenderViews(torch.autograd.Function):
         ids = torch.empty((len(views), h, w), dtype=torch.int32, device=cps.device)
         torch.cuda.synchronize()
         for i in range(len(views)):
-            renderer.read_device(i, out[i].data_ptr(), ids[i].data_ptr())
+            renderer.read_device(i, out[i].data_ptr(), ids[i].data_ptr(), h * w)
         ctx.renderer = renderer
+        ctx.serial = renderer.frame_serial()
         ctx.shape = cps.shape
         ctx.n_views = len(views)
         ctx.mark_non_differentiable(ids)
@@ -53,12 +59,23 @@ class _RenderViews(torch.autograd.Function):
         g_col = torch.empty((V, 4), dtype=torch.float32, device=grad_rgb.device)
         g_amp = torch.empty((V,), dtype=torch.float32, device=grad_rgb.device)
         torch.cuda.synchronize()
-        ctx.renderer.backward(dl.data_ptr(), g_cps.data_ptr(), g_col.data_ptr(), g_amp.data_ptr())
+        ctx.renderer.backward(ctx.serial, dl.data_ptr(), dl.numel(), g_cps.data_ptr(), g_cps.numel(),
+                              g_col.data_ptr(), g_amp.data_ptr(), V)
         return g_cps.sum(0), g_amp.sum(), g_col[:, :3].sum(0), None, None
+
+
+def check_views(views):
+    """Raise unless every (projection, modelview) view shares one viewport."""
+    if not views:
+        raise MicropolisError("render_views needs at least one view")
+    sizes = {tuple(p.viewport) for p, _ in views}
+    if len(sizes) != 1:
+        raise MicropolisError("render_views needs one viewport for every view, got %s" % sorted(sizes))
 
 
 def render_views(renderer, cps, amplitude, albedo, views):
     """-> (rgb (V,H,W,3) float32, ids (V,H,W) int32 visible micropolygon ids)."""
+    check_views(views)
     if not torch.is_tensor(amplitude):
         amplitude = torch.tensor(float(amplitude))
     return _RenderViews.apply(cps, amplitude, albedo, renderer, views)

@@ -26,6 +26,7 @@
 
 #include "../../include/micropolis.h"
 #include "mp_cpu_split.h"
+#include "mp_host.h"
 #include "mp_kernels.h"
 #include "mp_types.h"
 
@@ -140,6 +141,13 @@ struct mp_context {
     mp_stats stats;
     std::vector<int32_t> pass_counts;
     bool frame_valid = false;
+    uint64_t frame_serial = 0;   // the last finished frame (0: none); see mp_frame_serial
+    uint64_t serial_counter = 0;
+    // Per-patch DRAW counts and bucket cursors are zero between frames (the
+    // rank kernel clears them).  A launch sequence that stops with an error
+    // after the split has counted but before the rank ran leaves them stale;
+    // it sets this flag and the next frame clears them first.
+    bool counts_dirty = false;
     bool device_driven = false;  // the last frame can be replayed without host syncs
     hipGraphExec_t graph = nullptr;
     bool work_sharing = false;   // spill queue between split groups
@@ -222,7 +230,7 @@ static void invalidate_graph(mp_context* ctx)
 // ---------------------------------------------------------------------------
 // projection (src/base/Projection.cpp:23-50; glm::perspective, far 10000)
 
-static void make_view(const mp_projection& pr, const float clear[4], int subpixel_bits, DevView& v)
+void mp::make_view(const mp_projection& pr, const float clear[4], int subpixel_bits, DevView& v)
 {
     memset(&v, 0, sizeof(v));
     const int W = pr.width, H = pr.height;
@@ -261,7 +269,7 @@ static void make_view(const mp_projection& pr, const float clear[4], int subpixe
 
 // ---------------------------------------------------------------------------
 
-static int validate_config(const mp_config* c, std::string& why)
+int mp::validate_config(const mp_config* c, std::string& why)
 {
     if (c->reyes_patch_size != 8 && c->reyes_patch_size != 16 && c->reyes_patch_size != 32) {
         why = "reyes_patch_size must be 8, 16 or 32";
@@ -279,6 +287,27 @@ static int validate_config(const mp_config* c, std::string& why)
     if (c->window_width < 1 || c->window_height < 1) { why = "window_size must be positive"; return 0; }
     if (c->displacement && c->displacement_octaves < 1) { why = "displacement_octaves must be >= 1"; return 0; }
     return 1;
+}
+
+DevParams mp::device_params(const mp_config* cfg)
+{
+    DevParams p;
+    memset(&p, 0, sizeof(p));
+    p.patch_size = cfg->reyes_patch_size;
+    p.max_split_depth = cfg->max_split_depth;
+    p.cull_ribbon = cfg->cull_ribbon;
+    p.backface_culling = cfg->backface_culling;
+    p.subpixel_bits = cfg->subpixel_bits;
+    p.patches_per_pass = cfg->reyes_patches_per_pass;
+    p.displacement = cfg->displacement;
+    p.disp_frequency = cfg->displacement_frequency;
+    p.disp_seed = cfg->displacement_seed;
+    p.disp_octaves = cfg->displacement_octaves;
+    p.noise_cache = 0;        // hashing the lattice per use measured faster than the per-range LDS cache
+    p.noise_wave_cells = 2;   // lane window (0 per lane, 1 wave-uniform cell)
+    p.raster_chunk = 0;       // static range stride (see mp_tuning)
+    p.raster_one_wave = 1;    // N = 16: one wave per work-group (no waits between waves)
+    return p;
 }
 
 extern "C" int mp_config_defaults(mp_config* c)
@@ -314,10 +343,12 @@ extern "C" int mp_config_defaults(mp_config* c)
 
 extern "C" const char* mp_create_error(void) { return g_create_error.c_str(); }
 
+void mp::set_create_error(const std::string& msg) { g_create_error = msg; }
+
 // tab[k - 1] = smallest positive float whose display byte is at least k
 // (binary search over the bit patterns of positive floats, on which the
 // byte is non-decreasing); tab[255] = NaN, never passed by a comparison.
-static void display_thresholds(float tab[256])
+void mp::display_thresholds(float tab[256])
 {
     for (uint32_t k = 1; k <= 255; ++k) {
         uint32_t lo = 1u, hi = 0x7f800000u;   // smallest denormal .. +inf (byte 255)
@@ -356,17 +387,7 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     mp_context* ctx = new mp_context();
     ctx->cfg = *cfg;
     ctx->amplitude = cfg->displacement_amplitude;
-    DevParams& p = ctx->prm;
-    p.patch_size = cfg->reyes_patch_size;
-    p.max_split_depth = cfg->max_split_depth;
-    p.cull_ribbon = cfg->cull_ribbon;
-    p.backface_culling = cfg->backface_culling;
-    p.subpixel_bits = cfg->subpixel_bits;
-    p.patches_per_pass = cfg->reyes_patches_per_pass;
-    p.displacement = cfg->displacement;
-    p.disp_frequency = cfg->displacement_frequency;
-    p.disp_seed = cfg->displacement_seed;
-    p.disp_octaves = cfg->displacement_octaves;
+    ctx->prm = device_params(cfg);
 
     hipDeviceProp_t prop;
     if (hipGetDevice(&ctx->device) != hipSuccess ||
@@ -392,12 +413,7 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
         }
     }
     ctx->num_cus = prop.multiProcessorCount;
-    // hashing the lattice per use measured faster than the per-range LDS cache
-    ctx->prm.noise_cache = 0;
     if (const char* e = getenv("MP_NOISE_CACHE")) ctx->prm.noise_cache = atoi(e);
-    ctx->prm.noise_wave_cells = 2;   // lane window (0 per lane, 1 wave-uniform cell)
-    ctx->prm.raster_chunk = 0;   // static range stride (see mp_tuning)
-    ctx->prm.raster_one_wave = 1;   // N = 16: one wave per work-group (no waits between waves)
     if (const char* e = getenv("MP_RASTER_ONE_WAVE")) ctx->prm.raster_one_wave = atoi(e) != 0;
     if (const char* e = getenv("MP_RASTER_CHUNK")) ctx->prm.raster_chunk = atoi(e);
     if (const char* e = getenv("MP_NOISE_WAVE_CELLS")) ctx->prm.noise_wave_cells = atoi(e);
@@ -507,6 +523,7 @@ extern "C" int mp_load_patches(mp_context* ctx, uint64_t handle, const float* xy
     ctx->patches[handle] = ps;
     invalidate_graph(ctx);
     ctx->frame_valid = false;
+    ctx->frame_serial = ++ctx->serial_counter;
     return MP_OK;
 }
 
@@ -520,6 +537,7 @@ extern "C" int mp_unload_patches(mp_context* ctx, uint64_t handle)
     ctx->patches.erase(it);
     invalidate_graph(ctx);
     ctx->frame_valid = false;
+    ctx->frame_serial = ++ctx->serial_counter;
     return MP_OK;
 }
 
@@ -561,7 +579,13 @@ extern "C" int mp_draw_patches(mp_context* ctx, uint64_t handle, const float mv[
     if (!projection) return fail(ctx, MP_E_INVALID, "null projection");
     if (projection->width != ctx->cfg.window_width || projection->height != ctx->cfg.window_height)
         return fail(ctx, MP_E_INVALID, "projection viewport must equal the window size");
-    if (ctx->objects.empty()) ctx->view_projs[0] = *projection;
+    if (ctx->objects.empty()) {
+        ctx->view_projs[0] = *projection;
+    } else if (projection->fovy_deg != ctx->view_projs[0].fovy_deg ||
+               projection->near_plane != ctx->view_projs[0].near_plane) {
+        // one projection per frame (Scene::draw passes the active camera's)
+        return fail(ctx, MP_E_INVALID, "every draw of a frame must use the first draw's fovy and near plane");
+    }
     return add_object(ctx, 0, handle, mv, color);
 }
 
@@ -667,6 +691,17 @@ static CanonBuffers canon_buffers(mp_context* ctx)
     cb.part = ctx->d_part;
     cb.n_gpids = ctx->n_gpids;
     return cb;
+}
+
+// Clears per-patch counts and bucket cursors that an interrupted launch
+// sequence may have left non-zero (see mp_context::counts_dirty).
+static int clear_stale_counts(mp_context* ctx)
+{
+    if (!ctx->counts_dirty || !ctx->d_counts) return MP_OK;
+    HIPCHK(ctx, launch_fill_u32(ctx->d_counts, ctx->gpid_cap, 0u, ctx->stream));
+    HIPCHK(ctx, launch_fill_u32(ctx->d_cursor, ctx->gpid_cap, 0u, ctx->stream));
+    ctx->counts_dirty = false;
+    return MP_OK;
 }
 
 static int upload_tables(mp_context* ctx)
@@ -1061,6 +1096,8 @@ static int execute_frame(mp_context* ctx)
     if ((rc = upload_tables(ctx))) return rc;
     if ((rc = ensure_gpids(ctx, ctx->n_gpids))) return rc;
     if (ctx->list_cap == 0 && (rc = ensure_lists(ctx, std::max<uint32_t>(65536, 4 * ctx->total_roots)))) return rc;
+    if ((rc = clear_stale_counts(ctx))) return rc;
+    ctx->counts_dirty = true;   // until the frame has completed
     const int method = ctx->cfg.bound_n_split_method;
     const bool device_driven = (method == MP_BNS_LOCAL || method == MP_BNS_BALANCED);
     int64_t relaunches = 0, dropped = 0, max_patches = 0;
@@ -1122,6 +1159,11 @@ static int execute_frame(mp_context* ctx)
     HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.0f;
     hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    // the canonical order's scan must see exactly this frame's DRAW ranges
+    const uint32_t emitted = device_driven ? ctx->h_fc->draw_count : draw_total;
+    if ((ctx->h_fc->overflow & 16u) || ctx->h_fc->total_ranges != emitted)
+        return fail(ctx, MP_E_STATE, "per-patch range counts disagree with the draw list; frame discarded");
+    ctx->counts_dirty = false;
 
     mp_stats& st = ctx->stats;
     memset(&st, 0, sizeof(st));
@@ -1139,6 +1181,7 @@ static int execute_frame(mp_context* ctx)
     st.frame_ms = ms;
     ctx->device_driven = device_driven;
     ctx->frame_valid = true;
+    ctx->frame_serial = ++ctx->serial_counter;
     return MP_OK;
 }
 
@@ -1380,11 +1423,13 @@ extern "C" int mp_dump_trace(mp_context* ctx, const char* path)
     if (ctx->device_driven) {
         // one more run of the frame with an event between its stages
         hipEvent_t ev[5] = { nullptr, nullptr, nullptr, nullptr, nullptr };
-        int rc = MP_OK;
+        int rc = clear_stale_counts(ctx);
+        ctx->counts_dirty = true;
         for (auto& e : ev)
-            if (hipEventCreate(&e) != hipSuccess) rc = fail(ctx, MP_E_HIP, "hipEventCreate failed");
+            if (rc == MP_OK && hipEventCreate(&e) != hipSuccess) rc = fail(ctx, MP_E_HIP, "hipEventCreate failed");
         if (rc == MP_OK) rc = launch_device_frame(ctx, ev);
         if (rc == MP_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, MP_E_HIP, "sync failed");
+        if (rc == MP_OK) ctx->counts_dirty = false;
         // names from the reference's trace categories (tools/trace_summary.py):
         // split -> subdivision, canonical order (scan + scatter) -> subdivision,
         // fused dice/shade/sample -> sample, resolve (also clears) -> clear
@@ -1416,6 +1461,8 @@ extern "C" int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* 
     if (!ctx || n < 1) return MP_E_INVALID;
     if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame to replay");
     int rc;
+    if ((rc = clear_stale_counts(ctx))) return rc;
+    ctx->counts_dirty = true;   // until the replays have completed
     if (!ctx->device_driven) {
         if (use_graph) return fail(ctx, MP_E_INVALID, "graph replay needs a device-driven split method");
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -1458,6 +1505,7 @@ extern "C" int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* 
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     HIPCHK(ctx, hipMemcpy(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost));
     if (ctx->h_fc->overflow) return fail(ctx, MP_E_STATE, "replay overflowed the draw list");
+    ctx->counts_dirty = false;
     if (ms_per_frame) *ms_per_frame = (double)ms / n;
     return MP_OK;
 }
@@ -1498,10 +1546,13 @@ static int apply_tuning(mp_context* ctx, const mp_tuning& t)
     if (!ctx->frame_valid || !ctx->device_driven) return MP_OK;
     int rc;
     if ((rc = ensure_queues(ctx))) return rc;
+    if ((rc = clear_stale_counts(ctx))) return rc;
+    ctx->counts_dirty = true;
     if ((rc = launch_device_frame(ctx))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->h_fc->overflow) return fail(ctx, MP_E_STATE, "frame overflowed under this tuning");
+    ctx->counts_dirty = false;
     if (ctx->split_engine == 1 && (rc = update_level_launches(ctx, nullptr))) return rc;
     return MP_OK;
 }
@@ -1627,12 +1678,14 @@ extern "C" int mp_replay_stages(mp_context* ctx, int32_t n, double stage_ms[4])
     if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame to replay");
     if (!ctx->device_driven) return fail(ctx, MP_E_INVALID, "stage replay needs a device-driven split method");
     std::vector<hipEvent_t> ev((size_t)n * 5, nullptr);
-    int rc = MP_OK;
+    int rc = clear_stale_counts(ctx);
+    ctx->counts_dirty = true;
     for (auto& e : ev)
-        if (hipEventCreate(&e) != hipSuccess) { rc = fail(ctx, MP_E_HIP, "hipEventCreate failed"); break; }
+        if (rc == MP_OK && hipEventCreate(&e) != hipSuccess) { rc = fail(ctx, MP_E_HIP, "hipEventCreate failed"); break; }
     for (int i = 0; i < n && rc == MP_OK; ++i) rc = launch_device_frame(ctx, &ev[(size_t)i * 5]);
     double acc[4] = { 0, 0, 0, 0 };
     if (rc == MP_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, MP_E_HIP, "sync failed");
+    if (rc == MP_OK) ctx->counts_dirty = false;
     for (int i = 0; i < n && rc == MP_OK; ++i)
         for (int k = 0; k < 4; ++k) {
             float ms = 0.0f;
@@ -1646,12 +1699,29 @@ extern "C" int mp_replay_stages(mp_context* ctx, int32_t n, double stage_ms[4])
     return MP_OK;
 }
 
-extern "C" int mp_backward(mp_context* ctx, const float* dl_dcolor, float* grad_cps, float* grad_color,
-                           float* grad_amplitude)
+extern "C" int mp_frame_serial(mp_context* ctx, uint64_t* serial)
+{
+    if (!ctx || !serial) return MP_E_INVALID;
+    *serial = ctx->frame_valid ? ctx->frame_serial : 0u;
+    return MP_OK;
+}
+
+extern "C" int mp_backward(mp_context* ctx, uint64_t frame_serial, const float* dl_dcolor, size_t dl_floats,
+                           float* grad_cps, size_t grad_cps_floats, float* grad_color, float* grad_amplitude,
+                           int32_t n_objects)
 {
     if (!ctx) return MP_E_INVALID;
     if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame");
+    if (frame_serial != ctx->frame_serial)
+        return fail(ctx, MP_E_STATE, "the context rendered another frame (or its patches changed) since that frame");
     if (!dl_dcolor || !grad_cps || !grad_color || !grad_amplitude) return fail(ctx, MP_E_INVALID, "null pointer");
+    size_t pixels = 0;
+    for (int v = 0; v < ctx->n_views; ++v) pixels += (size_t)ctx->views[v].width * ctx->views[v].height;
+    if (dl_floats != 4 * pixels) return fail(ctx, MP_E_INVALID, "dl_dcolor must hold 4 floats per pixel of every view");
+    if (grad_cps_floats != ctx->grad_floats)
+        return fail(ctx, MP_E_INVALID, "grad_cps must hold n_patches*16*3 floats per object of the frame");
+    if (n_objects != (int32_t)ctx->objects.size())
+        return fail(ctx, MP_E_INVALID, "n_objects must equal the draws of the frame");
     if (ctx->cfg.reyes_patch_size != 8 && ctx->cfg.reyes_patch_size != 16)
         return fail(ctx, MP_E_INVALID, "mp_backward supports reyes_patch_size 8 and 16");
     for (const ObjectReq& o : ctx->objects)
@@ -1683,13 +1753,14 @@ extern "C" int mp_backward(mp_context* ctx, const float* dl_dcolor, float* grad_
     return MP_OK;
 }
 
-extern "C" int mp_batch_read_device(mp_context* ctx, int32_t view, float* rgba32, uint32_t* ids)
+extern "C" int mp_batch_read_device(mp_context* ctx, int32_t view, float* rgba32, uint32_t* ids, size_t n_pixels)
 {
     if (!ctx) return MP_E_INVALID;
     if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame");
     if (view < 0 || view >= ctx->n_views) return fail(ctx, MP_E_INVALID, "bad view index");
     const ViewBuffers& vb = ctx->views[view];
     const size_t np = (size_t)vb.width * vb.height;
+    if (n_pixels != np) return fail(ctx, MP_E_INVALID, "buffer size differs from the view's width * height");
     if (rgba32) {
         if (!vb.rgba32) return fail(ctx, MP_E_STATE, "write_rgba32 is off");
         HIPCHK(ctx, hipMemcpyAsync(rgba32, vb.rgba32, np * 16, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1709,5 +1780,6 @@ extern "C" int mp_update_patches_device(mp_context* ctx, uint64_t handle, const 
     if (n_points && !xyz) return fail(ctx, MP_E_INVALID, "null patch data");
     HIPCHK(ctx, launch_xyz_to_cps(xyz, it->second.d_cps, n_points, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->frame_serial = ++ctx->serial_counter;   // the last frame no longer matches its patches
     return MP_OK;
 }

@@ -980,7 +980,9 @@ __global__ __launch_bounds__(1024) void k_scan_counts(CanonBuffers cb, DevView* 
     }
     if (t == 0) {
         cb.offsets[n] = tot;
-        fc->total_ranges = tot;
+        // more counted ranges than the list holds: stale counts (flagged, clamped)
+        if (tot > cb.cap) atomicOr(&fc->overflow, 16u);
+        fc->total_ranges = min(tot, cb.cap);
     }
 }
 
@@ -1018,7 +1020,8 @@ __global__ __launch_bounds__(1024) void k_scan_parts(CanonBuffers cb, uint32_t n
     }
     if (threadIdx.x == 0) {
         cb.offsets[cb.n_gpids] = carry;
-        fc->total_ranges = carry;
+        if (carry > cb.cap) atomicOr(&fc->overflow, 16u);
+        fc->total_ranges = min(carry, cb.cap);
     }
 }
 
@@ -1087,10 +1090,10 @@ __global__ __launch_bounds__(256) void k_scatter(const DevObject* __restrict__ o
         uint32_t b = 0;
         if (lead_cnt) b = cb.offsets[g] + atomicAdd(&cb.cursor[g], lead_cnt);
         const uint32_t slot = (uint32_t)__shfl((int)b, my_leader) + my_rank;
-        if (valid) {
+        if (valid && slot < cb.cap) {
             cb.bucket[slot] = r;
             cb.keys[slot] = canonical_key(r);
-            cb.span[slot] = make_uint2(cb.offsets[g], cb.offsets[g + 1]);
+            cb.span[slot] = make_uint2(cb.offsets[g], min(cb.offsets[g + 1], cb.cap));
         }
     }
 }
@@ -1116,18 +1119,23 @@ __global__ __launch_bounds__(1024) void k_scan_scatter(const DevObject* __restri
         c[k] = i < n ? cb.counts[i] : 0u;
         sum += c[k];
     }
-    uint32_t total;
-    uint32_t run = block_exclusive_scan_1024(sum, lw, total);
+    uint32_t counted;
+    uint32_t run = block_exclusive_scan_1024(sum, lw, counted);
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
         const uint32_t i = t * 4u + k;
         if (i < n) s_off[i] = run;
         run += c[k];
     }
-    if (t == 0) s_off[n] = total;
+    if (t == 0) s_off[n] = counted;
     __syncthreads();
+    // more counted ranges than the list holds: stale counts (flagged, clamped)
+    const uint32_t total = min(counted, cb.cap);
     if (blockIdx.x == 0) {
-        if (t == 0) fc->total_ranges = total;
+        if (t == 0) {
+            fc->total_ranges = total;
+            if (counted > cb.cap) atomicOr(&fc->overflow, 16u);
+        }
         for (int v = (int)t; v < n_views; v += 1024) {
             const uint32_t b = s_off[views[v].first_gpid];
             views[v].range_base = b;
@@ -1159,10 +1167,10 @@ __global__ __launch_bounds__(1024) void k_scan_scatter(const DevObject* __restri
         uint32_t b = 0;
         if (lead_cnt) b = s_off[g] + atomicAdd(&cb.cursor[g], lead_cnt);
         const uint32_t slot = (uint32_t)__shfl((int)b, my_leader) + my_rank;
-        if (valid) {
+        if (valid && slot < cb.cap) {
             cb.bucket[slot] = r;
             cb.keys[slot] = canonical_key(r);
-            cb.span[slot] = make_uint2(s_off[g], s_off[g + 1]);
+            cb.span[slot] = make_uint2(s_off[g], min(s_off[g + 1], cb.cap));
         }
     }
 }
@@ -1194,6 +1202,9 @@ __global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, const FrameCounte
         RangeRec r;
         if (valid) {
             sp = cb.span[j];
+            // inside the lists even for spans a stale count produced
+            sp.x = min(sp.x, cb.cap);
+            sp.y = min(max(sp.y, sp.x), cb.cap);
             key = cb.keys[j];
             r = cb.bucket[j];
         }
@@ -1228,7 +1239,7 @@ __global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, const FrameCounte
             } else {
                 for (uint32_t k = sp.x; k < sp.y; ++k) rank += cb.keys[k] < key ? 1u : 0u;
             }
-            cb.canon[sp.x + rank] = r;
+            if (sp.x + rank < cb.cap) cb.canon[sp.x + rank] = r;
         }
         __syncthreads();   // s_key / s_lo reused by the next 256 ranges
     }

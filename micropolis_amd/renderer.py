@@ -212,11 +212,23 @@ class Renderer:
         return {name: getattr(st, name) for name, _ in mp_stats._fields_}
 
     # -- differentiable extension (device pointers; see include/micropolis.h) --
-    def backward(self, dl_dcolor_ptr, grad_cps_ptr, grad_color_ptr, grad_amp_ptr):
-        self._check(self._lib.mp_backward(self._ctx, dl_dcolor_ptr, grad_cps_ptr, grad_color_ptr, grad_amp_ptr))
+    def frame_serial(self):
+        """Serial number of the last finished frame or batch (0: none)."""
+        s = C.c_uint64()
+        self._check(self._lib.mp_frame_serial(self._ctx, C.byref(s)))
+        return s.value
 
-    def read_device(self, view, rgba32_ptr=None, ids_ptr=None):
-        self._check(self._lib.mp_batch_read_device(self._ctx, int(view), rgba32_ptr, ids_ptr))
+    def backward(self, frame_serial, dl_dcolor_ptr, dl_floats, grad_cps_ptr, grad_cps_floats, grad_color_ptr,
+                 grad_amp_ptr, n_objects):
+        """mp_backward for the frame `frame_serial`; the sizes are the caller's
+        buffer sizes in floats (grad_color: 4 per object, grad_amp: 1)."""
+        self._check(self._lib.mp_backward(self._ctx, int(frame_serial), dl_dcolor_ptr, int(dl_floats),
+                                          grad_cps_ptr, int(grad_cps_floats), grad_color_ptr, grad_amp_ptr,
+                                          int(n_objects)))
+
+    def read_device(self, view, rgba32_ptr=None, ids_ptr=None, n_pixels=0):
+        """Device copy of a view's RGBA32F colour / ids into buffers of n_pixels pixels."""
+        self._check(self._lib.mp_batch_read_device(self._ctx, int(view), rgba32_ptr, ids_ptr, int(n_pixels)))
 
     def update_patches_device(self, handle, xyz_ptr, n_points):
         self._check(self._lib.mp_update_patches_device(self._ctx, int(handle), xyz_ptr, int(n_points)))

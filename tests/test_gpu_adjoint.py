@@ -110,3 +110,56 @@ def test_adjoint_matches_central_differences(oracle):
         c2 = (f(e / 2) - f(-e / 2)) / e
         fd = (4 * c2 - c1) / 3
         assert abs(fd - g[i]) <= 1e-3 * abs(fd), (i, fd, g[i])
+
+
+def test_backward_refuses_a_frame_rendered_since(oracle):
+    """Two forwards on one renderer, then the first one's backward: the
+    library refuses (the visibility and control points it would use are the
+    second frame's), and the second forward's backward still works."""
+    from micropolis_amd.autograd import render_views
+    from micropolis_amd.renderer import MicropolisError
+    w = scenes.c5_views(n_views=1, width=96, height=64)
+    views = [(v.projection, v.objects[0].matrix) for v in w.views]
+    base = w.views[0].objects[0]
+    r = Renderer(**dict(w.options, write_rgba32=1))
+    cps = torch.tensor(base.patches, device="cuda").requires_grad_(True)
+    albedo = torch.tensor(base.color[:3], device="cuda")
+    rgb1, _ = render_views(r, cps, 0.02, albedo, views)
+    rgb2, _ = render_views(r, cps, 0.02, albedo, views)
+    with pytest.raises(MicropolisError, match="another frame"):
+        rgb1.sum().backward()
+    rgb2.sum().backward()
+    assert cps.grad is not None and torch.isfinite(cps.grad).all()
+    r.close()
+
+
+def test_backward_and_readback_check_buffer_sizes():
+    w = scenes.c5_views(n_views=2, width=96, height=64)
+    base = w.views[0].objects[0]
+    r = Renderer(**dict(w.options, write_rgba32=1))
+    r.load_patches(base.handle, base.patches, base.patch_type)
+    r.batch_begin([v.projection for v in w.views])
+    for vi, v in enumerate(w.views):
+        r.batch_draw(vi, base.handle, v.objects[0].matrix, base.color)
+    r.batch_finish()
+    from micropolis_amd.renderer import MicropolisError
+    out = torch.empty((64, 96, 4), device="cuda")
+    with pytest.raises(MicropolisError, match="width"):
+        r.read_device(0, out.data_ptr(), None, 64 * 95)
+    r.read_device(0, out.data_ptr(), None, 64 * 96)
+    s = r.frame_serial()
+    dl = torch.zeros((2, 64, 96, 4), device="cuda")
+    g = torch.empty((2,) + base.patches.shape, device="cuda")
+    gc = torch.empty((2, 4), device="cuda")
+    ga = torch.empty((2,), device="cuda")
+    with pytest.raises(MicropolisError, match="dl_dcolor"):
+        r.backward(s, dl.data_ptr(), dl.numel() - 4, g.data_ptr(), g.numel(), gc.data_ptr(), ga.data_ptr(), 2)
+    with pytest.raises(MicropolisError, match="grad_cps"):
+        r.backward(s, dl.data_ptr(), dl.numel(), g.data_ptr(), g.numel() // 2, gc.data_ptr(), ga.data_ptr(), 2)
+    with pytest.raises(MicropolisError, match="n_objects"):
+        r.backward(s, dl.data_ptr(), dl.numel(), g.data_ptr(), g.numel(), gc.data_ptr(), ga.data_ptr(), 1)
+    r.backward(s, dl.data_ptr(), dl.numel(), g.data_ptr(), g.numel(), gc.data_ptr(), ga.data_ptr(), 2)
+    torch.cuda.synchronize()
+    assert float(g.abs().max()) == 0.0   # zero loss gradient -> zero control-point gradient
+    r.close()
+

@@ -166,3 +166,23 @@ def test_one_renderer_across_patch_counts(oracle):
         assert_frame_equal(got, ref, w.name)
         r.unload_patches(o.handle)
     r.close()
+
+
+def test_draws_of_one_frame_share_the_projection():
+    """A later draw with another fovy or near plane is refused (one
+    projection per frame, as Scene::draw passes the active camera's)."""
+    from micropolis_amd.renderer import MicropolisError
+    w = scenes.c1_sphere()
+    o = w.views[0].objects[0]
+    r = Renderer(**dict(w.options, window_size=(64, 48)))
+    r.load_patches(o.handle, o.patches, o.patch_type)
+    r.prepare()
+    r.draw_patches(o.handle, o.matrix, Projection(60.0, 0.01, (64, 48)), o.color)
+    r.draw_patches(o.handle, o.matrix, Projection(60.0, 0.01, (64, 48)), o.color)
+    with pytest.raises(MicropolisError, match="fovy"):
+        r.draw_patches(o.handle, o.matrix, Projection(45.0, 0.01, (64, 48)), o.color)
+    with pytest.raises(MicropolisError, match="fovy"):
+        r.draw_patches(o.handle, o.matrix, Projection(60.0, 0.1, (64, 48)), o.color)
+    r.finish()
+    assert r.stats()["diced_ranges"] > 0
+    r.close()
