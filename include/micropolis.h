@@ -322,6 +322,29 @@ int mp_batch_read_device(mp_context* ctx, int32_t view, float* rgba32, uint32_t*
  * floats, same count as loaded). */
 int mp_update_patches_device(mp_context* ctx, uint64_t handle, const float* xyz, size_t n_points);
 
+/* ---- allocation-free frame (no reference counterpart) ----
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * allocation-free variant of the device-driven frame (mp_create +
+ * mp_load_patches + mp_prepare + mp_draw_patches + mp_finish +
+ * mp_read_framebuffer for one object, LOCAL or BALANCED): no context, no
+ * hipMalloc / hipFree.  Split queues, range lists, per-patch counts, the
+ * Lambert store and the visibility keys live in one 64 MiB device array of
+ * the library (per device, held until the process ends); a frame that does
+ * not fit it (window keys + patches + DRAW ranges) is refused with
+ * MP_E_INVALID.  Every call on one device shares that array: calls that
+ * overlap on one device (other host threads or streams) race.
+ *   xyz                : device pointer, n_points * 3 floats
+ *   rgba8, depth, ids  : device pointers, width * height each (layout of
+ *                        mp_read_framebuffer)
+ *   stream             : hipStream_t to run on (NULL: the null stream);
+ *                        the call waits for it before returning
+ * The image equals mp_finish's for the same configuration and draw.  On
+ * failure mp_create_error() holds the reason. */
+int mp_render_noalloc(const mp_config* cfg, const float* xyz, size_t n_points, int32_t patch_type,
+                      const float mv[16], const mp_projection* projection, const float color[4],
+                      uint8_t* rgba8, float* depth, uint32_t* ids, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,6 +110,8 @@ EXPORTS = [
     ("mp_get_tuning", C.c_int, [_P, C.POINTER(mp_tuning)]),
     ("mp_set_tuning", C.c_int, [_P, C.POINTER(mp_tuning)]),
     ("mp_autotune", C.c_int, [_P, C.c_int32, C.POINTER(mp_tuning)]),
+    ("mp_render_noalloc", C.c_int, [C.POINTER(mp_config), _P, C.c_size_t, C.c_int32, _FP,
+                                    C.POINTER(mp_projection), _FP, _P, _P, _P, _P]),
     ("mp_cpu_bound_n_split", C.c_int, [C.POINTER(mp_config), _FP, C.c_size_t, C.c_int32, _FP,
                                        C.POINTER(mp_projection), C.POINTER(mp_range_record), C.c_int64,
                                        C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int32,

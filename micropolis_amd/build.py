@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmicropolis_hip.so")
-SOURCES = ["csrc/mp_kernels.hip", "csrc/mp_api.hip", "csrc/mp_cpu_split.cpp"]
+SOURCES = ["csrc/mp_kernels.hip", "csrc/mp_api.hip", "csrc/mp_noalloc.hip", "csrc/mp_cpu_split.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # IEEE-strict f32: no contraction into FMA; '/' and sqrt stay correctly rounded
