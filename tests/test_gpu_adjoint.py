@@ -34,7 +34,7 @@ def setup_problem(n_views=2, width=480, height=272):
     loss = (((rgb - target) ** 2) * mask).sum()
     loss.backward()
     return dict(w=w, r=r, tgt_r=tgt_r, cps=cps, amp=amp, albedo=albedo, ids=ids, target=target, loss=loss,
-                opts=opts, views=views)
+                opts=opts, views=views, rgb=rgb.detach(), mask=mask)
 
 
 def fp64_reference(oracle, pb):
@@ -162,4 +162,87 @@ def test_backward_and_readback_check_buffer_sizes():
     torch.cuda.synchronize()
     assert float(g.abs().max()) == 0.0   # zero loss gradient -> zero control-point gradient
     r.close()
+
+
+def test_adjoint_at_the_c5_size(oracle):
+    """BASELINE configuration 5 at its stated size: 32 views of the displaced
+    torus at 960x540, loss = L2 to a target rendered with control points
+    + N(0, 0.01), amplitude 0.025 and albedo + 0.1.  The GPU gradients w.r.t.
+    all 4096x16x3 control points, the amplitude and the albedo against the
+    FP64 oracle over all 32 views (<= 1e-3 relative), and the per-view
+    gradients of mp_backward on four views against Richardson central
+    differences (eps 1e-3) of the FP64 fixed-visibility loss on the largest
+    control-point coordinates, the amplitude and the three albedo channels."""
+    pb = setup_problem(n_views=32, width=960, height=540)
+    loss64, g_cps, g_alb, g_amp = fp64_reference(oracle, pb)
+    print("c5: loss gpu %.6f fp64 %.6f" % (float(pb["loss"]), loss64), flush=True)
+    assert abs(float(pb["loss"]) - loss64) <= 1e-4 * loss64
+    assert rel(pb["cps"].grad.cpu().numpy(), g_cps) <= 1e-3
+    assert rel(pb["albedo"].grad.cpu().numpy(), g_alb) <= 1e-3
+    assert abs(float(pb["amp"].grad) - g_amp) <= 1e-3 * abs(g_amp)
+
+    # per-view gradients straight from mp_backward (the autograd node sums them)
+    w, r = pb["w"], pb["r"]
+    V = len(w.views)
+    dl = torch.zeros(pb["rgb"].shape[:-1] + (4,), device="cuda")
+    dl[..., :3] = 2.0 * (pb["rgb"] - pb["target"]) * pb["mask"]
+    P = pb["cps"].detach().cpu().numpy()
+    g_v = torch.empty((V,) + P.shape, device="cuda")
+    g_c = torch.empty((V, 4), device="cuda")
+    g_a = torch.empty((V,), device="cuda")
+    torch.cuda.synchronize()
+    r.backward(r.frame_serial(), dl.data_ptr(), dl.numel(), g_v.data_ptr(), g_v.numel(), g_c.data_ptr(),
+               g_a.data_ptr(), V)
+    sub = [0, 9, 17, 26]
+    gv = g_v[sub].sum(0).cpu().numpy().ravel()
+    gc = g_c[sub, :3].sum(0).cpu().numpy()
+    ga = float(g_a[sub].sum())
+    # the per-view outputs sum to the autograd gradient
+    assert rel(g_v.sum(0).cpu().numpy(), pb["cps"].grad.cpu().numpy()) <= 1e-5
+
+    ranges = r.ranges()
+    prm = oracle.make_params(pb["opts"])
+    ids = pb["ids"].cpu().numpy().view(np.uint32)
+    tgt = pb["target"].cpu().numpy()
+    amp0 = float(pb["amp"])
+    alb0 = pb["albedo"].detach().cpu().numpy().astype(np.float64)
+    per_view = []
+    for vi in sub:
+        rv = ranges[ranges["object"] == vi]
+        rr = np.zeros(len(rv), oracle.RANGE_DTYPE)
+        for f in ("pid", "depth", "umin", "vmin", "umax", "vmax"):
+            rr[f] = rv[f]
+        per_view.append((rr, w.views[vi].objects[0].matrix, ids[vi], tgt[vi]))
+
+    def L(x=P, amp=amp0, alb=alb0):
+        return sum(oracle.adjoint_f64(x, rr, mv, prm, amp, alb, iv, tv, grads=False)
+                   for rr, mv, iv, tv in per_view)
+
+    def richardson(f, e):
+        c1 = (f(e) - f(-e)) / (2 * e)
+        c2 = (f(e / 2) - f(-e / 2)) / e
+        return (4 * c2 - c1) / 3
+
+    for i in np.argsort(np.abs(gv))[-4:]:
+        x0 = np.float32(P.ravel()[i])
+        e = float(np.float32(x0 + np.float32(1e-3)) - x0)
+
+        def fx(h, i=i):
+            x = P.astype(np.float64).ravel()
+            x[i] += h
+            return L(x=x.reshape(P.shape).astype(np.float32))
+        fd = richardson(fx, e)
+        print("c5 cd: cp %d fd %.6g gpu %.6g" % (i, fd, gv[i]), flush=True)
+        assert abs(fd - gv[i]) <= 1e-3 * abs(fd), (i, fd, gv[i])
+    fd = richardson(lambda h: L(amp=amp0 + h), 1e-3)
+    print("c5 cd: amplitude fd %.6g gpu %.6g" % (fd, ga), flush=True)
+    assert abs(fd - ga) <= 1e-3 * abs(fd)
+    for c in range(3):
+        def fa(h, c=c):
+            a = alb0.copy()
+            a[c] += h
+            return L(alb=a)
+        fd = richardson(fa, 1e-3)
+        print("c5 cd: albedo[%d] fd %.6g gpu %.6g" % (c, fd, gc[c]), flush=True)
+        assert abs(fd - gc[c]) <= 1e-3 * abs(fd)
 
