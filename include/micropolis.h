@@ -174,6 +174,17 @@ typedef struct mp_range_record {
 int mp_read_ranges(mp_context* ctx, mp_range_record* records, int64_t capacity,
                    int64_t* count);
 
+/* Debug tap: the micropolygon grids of canonical ranges [first_range,
+ * first_range + n_ranges) of the last frame, as the dice stage computes them
+ * (kernels/dice.cl:75-79): per range (N+1)^2 points, index nu + nv*(N+1),
+ *   pos    : eye-space position x, y, z, w (4 floats per point)
+ *   pxlpos : sub-pixel coordinate x, y (2 int32 per point)
+ *   depth  : p.z / p.w (1 float per point)
+ * Range indices are those of mp_read_ranges.  Host pointers; allocates a
+ * temporary device buffer. */
+int mp_read_grids(mp_context* ctx, int64_t first_range, int64_t n_ranges, float* pos,
+                  int32_t* pxlpos, float* depth);
+
 /* Debug tap: per-pass DRAW counts of the last frame (BREADTH: per split
  * level; BOUNDED and CPU: per batch of reyes_patches_per_pass; LOCAL/BALANCED:
  * the DRAW total in slices of reyes_patches_per_pass). */

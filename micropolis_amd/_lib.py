@@ -90,6 +90,7 @@ EXPORTS = [
     ("mp_finish", C.c_int, [_P]),
     ("mp_read_framebuffer", C.c_int, [_P, _P, _P, _P, _P]),
     ("mp_read_ranges", C.c_int, [_P, C.POINTER(mp_range_record), C.c_int64, C.POINTER(C.c_int64)]),
+    ("mp_read_grids", C.c_int, [_P, C.c_int64, C.c_int64, _P, _P, _P]),
     ("mp_read_pass_counts", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]),
     ("mp_get_stats", C.c_int, [_P, C.POINTER(mp_stats)]),
     ("mp_set_displacement_amplitude", C.c_int, [_P, C.c_float]),

@@ -1344,6 +1344,35 @@ extern "C" int mp_read_ranges(mp_context* ctx, mp_range_record* records, int64_t
     return MP_OK;
 }
 
+extern "C" int mp_read_grids(mp_context* ctx, int64_t first_range, int64_t n_ranges, float* pos,
+                             int32_t* pxlpos, float* depth)
+{
+    if (!ctx) return MP_E_INVALID;
+    if (!ctx->frame_valid) return fail(ctx, MP_E_STATE, "no finished frame");
+    const int64_t total = ctx->h_fc->total_ranges;
+    if (first_range < 0 || n_ranges < 0 || first_range + n_ranges > total)
+        return fail(ctx, MP_E_INVALID, "range interval outside the frame's ranges");
+    if (!pos || !pxlpos || !depth) return fail(ctx, MP_E_INVALID, "null output");
+    if (n_ranges == 0) return MP_OK;
+    const size_t G = (size_t)(ctx->cfg.reyes_patch_size + 1) * (ctx->cfg.reyes_patch_size + 1);
+    const size_t pts = (size_t)n_ranges * G;
+    char* d = nullptr;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (hipMalloc((void**)&d, pts * (16 + 8 + 4)) != hipSuccess) return fail(ctx, MP_E_NOMEM, "grid tap buffer");
+    float4* d_pos = reinterpret_cast<float4*>(d);
+    int2* d_pxl = reinterpret_cast<int2*>(d + pts * 16);
+    float* d_dep = reinterpret_cast<float*>(d + pts * 24);
+    hipError_t e = launch_read_grids(ctx->cfg.reyes_patch_size, ctx->d_canon, ctx->d_objs, ctx->d_views, ctx->prm,
+                                     (uint32_t)first_range, (uint32_t)n_ranges, d_pos, d_pxl, d_dep, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(pos, d_pos, pts * 16, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(pxlpos, d_pxl, pts * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(depth, d_dep, pts * 4, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(ctx, MP_E_HIP, std::string("grid tap: ") + hipGetErrorString(e));
+    return MP_OK;
+}
+
 extern "C" int mp_read_pass_counts(mp_context* ctx, int32_t* counts, int32_t capacity, int32_t* n_passes)
 {
     if (!ctx) return MP_E_INVALID;

@@ -100,6 +100,11 @@ hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
 
 hipError_t launch_init_roots(int obj, uint32_t n, RangeRec* out, hipStream_t s);
 
+// debug tap: the diced grids of canonical ranges [first, first + n)
+hipError_t launch_read_grids(int patch_size, const RangeRec* canon, const DevObject* objs, const DevView* views,
+                             const DevParams& prm, uint32_t first, uint32_t n, float4* pos, int2* pxl, float* dep,
+                             hipStream_t s);
+
 // reverse pass of the differentiable extension
 hipError_t launch_adjoint(int patch_size, const RangeRec* canon, const DevObject* objs,
                           const DevView* views, int n_views, int max_pixels, const DevParams& prm,

@@ -1479,6 +1479,57 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
     }
 }
 
+// Dice one range (dice.cl:38-79) into LDS: eye-space position, sub-pixel
+// coordinate and depth of each of the (N+1)^2 grid points, displaced by the
+// value noise in MODE 1 (the displacement normal's column sums staged in
+// s_col first).  s_cp holds the range's control points.  Every lane of each
+// of the NT / 64 waves runs every pass (the lane-window noise exchanges
+// values across the wave); lanes past the last point repeat it and store
+// nothing.  nw: lattice windows the calling wave keeps across calls.
+template <int N, int MODE, int NT, typename PT>
+MP_DEV void dice_grid(const float4* __restrict__ s_cp, float4* __restrict__ s_col, const RangeRec& rec,
+                      const DevObject& ob, const DevView& vw, const DevParams& prm, PT* __restrict__ s_pos,
+                      int2* __restrict__ s_pxl, float* __restrict__ s_dep, NoiseWindows& nw)
+{
+    constexpr int G1 = N + 1;
+    constexpr int G = G1 * G1;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const bool disp_cols = MODE == 1 && ob.type == MP_TYPE_BEZIER;
+    if (disp_cols) {
+        // displacement normal: column sums once per u column
+        for (int t = tid; t < G1 * 8; t += NT) {
+            const int c = t >> 3;
+            const float uc = rec.umin + (rec.umax - rec.umin) * ((float)c / (float)N);
+            s_col[t] = normal_column(s_cp, uc, t & 7);
+        }
+        __syncthreads();
+    }
+    for (int gb = wave * 64; gb < G; gb += NT) {
+        // control points re-read from LDS each pass (registers for the kept
+        // lattice windows)
+        asm volatile("" ::: "memory");
+        const int g = min(gb + lane, G - 1);
+        const int nu = g % G1, nv = g / G1;
+        const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
+        const float uu = rec.umin + (rec.umax - rec.umin) * tu;
+        const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
+        float P[4], e[4], q[4];
+        if (disp_cols) displaced_point_cols(s_cp, s_col + 8 * nu, uu, vv, prm, ob.disp_amplitude, P, true, &nw);
+        else if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
+        else eval_patch(s_cp, ob.type, uu, vv, P);
+        mat4_mul(ob.mv, P, e);
+        mat4_mul(vw.proj, e, q);
+        const int32_t px = f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2));
+        const int32_t py = f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2));
+        if (gb + lane < G) {
+            set_pos(s_pos[g], e[0], e[1], e[2], e[3]);
+            s_pxl[g] = make_int2(px, py);
+            s_dep[g] = q[2] / q[3];
+        }
+    }
+}
+
 // NT threads per work-group: N*N (one wave per 8x8 block of quads) or 64
 // (one wave dices the whole grid and shades the blocks in turn: no waiting
 // between waves at the range's barriers).
@@ -1569,43 +1620,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const DevView& vw = views[ob.view];
 
         // ---- dice (dice.cl:38-79) ----
-        const bool disp_cols = MODE == 1 && ob.type == MP_TYPE_BEZIER;
-        if (disp_cols) {
-            // displacement normal: column sums once per u column
-            for (int t = tid; t < G1 * 8; t += NT) {
-                const int c = t >> 3;
-                const float uc = rec.umin + (rec.umax - rec.umin) * ((float)c / (float)N);
-                s_col[t] = normal_column(s_cp, uc, t & 7);
-            }
-            __syncthreads();
-        }
         if (MODE != 2 || ob.type != MP_TYPE_BEZIER) {
-            // wave-uniform trip count with every lane active (the lane-window
-            // noise exchanges values across the wave); lanes past the last
-            // point repeat it and store nothing
-            for (int gb = wave * 64; gb < G; gb += NT) {
-                // control points re-read from LDS each pass (registers for the
-                // kept lattice windows)
-                asm volatile("" ::: "memory");
-                const int g = min(gb + lane, G - 1);
-                const int nu = g % G1, nv = g / G1;
-                const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
-                const float uu = rec.umin + (rec.umax - rec.umin) * tu;
-                const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
-                float P[4], e[4], q[4];
-                if (disp_cols) displaced_point_cols(s_cp, s_col + 8 * nu, uu, vv, prm, ob.disp_amplitude, P, true, &nw);
-                else if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
-                else eval_patch(s_cp, ob.type, uu, vv, P);
-                mat4_mul(ob.mv, P, e);
-                mat4_mul(vw.proj, e, q);
-                const int32_t px = f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2));
-                const int32_t py = f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2));
-                if (gb + lane < G) {
-                    set_pos(s_pos[g], e[0], e[1], e[2], e[3]);
-                    s_pxl[g] = make_int2(px, py);
-                    s_dep[g] = q[2] / q[3];
-                }
-            }
+            dice_grid<N, MODE == 2 ? 0 : MODE, NT>(s_cp, s_col, rec, ob, vw, prm, s_pos, s_pxl, s_dep, nw);
         } else if constexpr (MODE == 2) {
             // displaced: surface point and normal, then the value noise with
             // each octave's lattice values hashed once per range into LDS
@@ -1788,6 +1804,67 @@ hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject*
     case 32: launch_raster_n<32, 1024>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// Debug tap: the diced grids of canonical ranges [first, first + n) of the
+// last frame, computed by dice_grid as the raster does (one 64-lane wave per
+// range), written as eye-space position (x, y, z, w), sub-pixel coordinate
+// and depth per grid point, range-major, point index nu + nv * (N + 1)
+// (dice.cl:21-24, 75-79).
+template <int N, int MODE>
+__global__ __launch_bounds__(64) void k_read_grids(const RangeRec* __restrict__ canon,
+                                                   const DevObject* __restrict__ objs,
+                                                   const DevView* __restrict__ views, DevParams prm,
+                                                   uint32_t first, uint32_t n, float4* __restrict__ pos,
+                                                   int2* __restrict__ pxl, float* __restrict__ dep)
+{
+    constexpr int G1 = N + 1;
+    constexpr int G = G1 * G1;
+    __shared__ float4 s_cp[20];
+    __shared__ float4 s_pos[G];
+    __shared__ int2 s_pxl[G];
+    __shared__ float s_dep[G];
+    __shared__ float4 s_col[MODE == 1 ? G1 * 8 : 1];
+    const int tid = threadIdx.x;
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const RangeRec rec = canon[first + k];
+        const DevObject& ob = objs[rec.obj_depth & 0xffffffu];
+        const int ncp = ob.type == MP_TYPE_GREGORY ? 20 : 16;
+        if (tid < ncp) s_cp[tid] = ldg4(ob.cps + (size_t)rec.pid * ncp + tid);
+        __syncthreads();
+        NoiseWindows nw;
+        nw.reset();
+        dice_grid<N, MODE, 64>(s_cp, s_col, rec, ob, views[ob.view], prm, s_pos, s_pxl, s_dep, nw);
+        __syncthreads();
+        for (int g = tid; g < G; g += 64) {
+            pos[(size_t)k * G + g] = s_pos[g];
+            pxl[(size_t)k * G + g] = s_pxl[g];
+            dep[(size_t)k * G + g] = s_dep[g];
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_read_grids(int patch_size, const RangeRec* canon, const DevObject* objs, const DevView* views,
+                             const DevParams& prm, uint32_t first, uint32_t n, float4* pos, int2* pxl, float* dep,
+                             hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const dim3 grid(std::min<uint32_t>(n, 4096u)), block(64);
+    const bool disp = prm.displacement != 0;
+#define MP_GRIDS(NN)                                                                                        \
+    if (disp) hipLaunchKernelGGL((k_read_grids<NN, 1>), grid, block, 0, s, canon, objs, views, prm, first, n, \
+                                 pos, pxl, dep);                                                            \
+    else hipLaunchKernelGGL((k_read_grids<NN, 0>), grid, block, 0, s, canon, objs, views, prm, first, n, pos,  \
+                            pxl, dep);
+    switch (patch_size) {
+    case 8: MP_GRIDS(8) break;
+    case 16: MP_GRIDS(16) break;
+    case 32: MP_GRIDS(32) break;
+    default: return hipErrorInvalidValue;
+    }
+#undef MP_GRIDS
     return hipGetLastError();
 }
 

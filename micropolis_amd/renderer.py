@@ -199,6 +199,18 @@ class Renderer:
                                                 ("umax", "<f4"), ("vmax", "<f4")]))
         return a[: n.value].copy()
 
+    def grids(self, first, count):
+        """Diced grids of canonical ranges [first, first + count): eye-space
+        pos (count, (N+1)^2, 4), sub-pixel pxlpos (count, (N+1)^2, 2) and
+        depth (count, (N+1)^2)."""
+        G = (self.cfg.reyes_patch_size + 1) ** 2
+        pos = np.empty((count, G, 4), np.float32)
+        pxl = np.empty((count, G, 2), np.int32)
+        dep = np.empty((count, G), np.float32)
+        self._check(self._lib.mp_read_grids(self._ctx, int(first), int(count), pos.ctypes.data, pxl.ctypes.data,
+                                            dep.ctypes.data))
+        return pos, pxl, dep
+
     def pass_counts(self):
         n = C.c_int32()
         self._check(self._lib.mp_read_pass_counts(self._ctx, None, 0, C.byref(n)))

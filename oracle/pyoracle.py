@@ -253,3 +253,83 @@ def adjoint_f64(patches, ranges, mv, prm, amplitude, albedo, ids, target_rgb, gr
     if not grads:
         return loss
     return loss, g_cps, g_alb, float(g_amp[0])
+
+
+def render_threaded(workload, threads):
+    """render() on `threads` host threads (LOCAL / BALANCED workloads: the
+    DRAW set is the union of the per-patch sets).  Each thread splits every
+    T-th patch of an object and rasterises a contiguous slice
+    of the canonical ranges into its own key buffer; the buffers are reduced
+    with a minimum (the visibility keys are order independent) and resolved
+    once.  Returns the same per-view dicts as render() (no pass counts)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    lib = load()
+    opts = defaults_from_options(workload.options)
+    meth = opts["bound_n_split_method"]
+    if meth not in ("LOCAL", "BALANCED"):
+        raise ValueError("render_threaded splits LOCAL / BALANCED workloads")
+    prm = make_params(opts)
+    N = prm.patch_size
+    limit = float(opts["bound_n_split_limit"])
+    T = max(1, int(threads))
+    results = []
+    with ThreadPoolExecutor(max_workers=T) as pool:
+        for v in workload.views:
+            pr = v.projection
+            W, H = pr.viewport
+            view = make_view(pr.fovy, pr.near, W, H, opts["clear_color"])
+            all_ranges, colors, n_bounds = [], [], 0
+            for ob in v.objects:
+                cps = as_cps(ob.patches)
+                P = cps.shape[0]
+
+                def split_slice(t, cps=cps, ob=ob, P=P):
+                    # patches t, t + T, ...: the deep split trees of neighbouring
+                    # patches spread over the threads
+                    idx = np.arange(t, P, T)
+                    if idx.size == 0:
+                        return np.zeros(0, RANGE_DTYPE), 0
+                    sub = np.ascontiguousarray(cps[idx])
+                    r, _, nb, _ = split(meth, sub, ob.patch_type, ob.matrix, view, prm, limit)
+                    r["pid"] = idx[r["pid"]]
+                    return r, nb
+                parts = list(pool.map(split_slice, range(T)))
+                r = canonical_sort(np.concatenate([p[0] for p in parts]))
+                n_bounds += sum(p[1] for p in parts)
+                all_ranges.append((ob, cps, r))
+                colors.append(np.repeat(ob.color.reshape(1, 4), len(r), axis=0))
+            total = sum(len(x[2]) for x in all_ranges)
+            quad = np.zeros(max(1, total) * N * N, np.float32)
+            size = lib.or_framebuffer_size(W, H)
+            jobs = []
+            base = 0
+            for ob, cps, r in all_ranges:
+                cuts = np.linspace(0, len(r), T + 1).astype(int)
+                for a, b in zip(cuts[:-1], cuts[1:]):
+                    if b > a:
+                        jobs.append((ob, cps, r, a, b, base))
+                base += len(r)
+
+            def raster(job):
+                ob, cps, r, a, b, base = job
+                fb = np.full(size, EMPTY_KEY, np.uint64)
+                chunk = np.ascontiguousarray(r[a:b])
+                lib.or_render(fptr(cps), ob.patch_type, chunk.ctypes.data, b - a, base + a,
+                              fptr(colmajor(ob.matrix)), C.byref(view), C.byref(prm), fb.ctypes.data,
+                              quad.ctypes.data)
+                return fb
+            fbs = list(pool.map(raster, jobs))
+            fb = np.minimum.reduce(fbs) if fbs else np.full(size, EMPTY_KEY, np.uint64)
+            fb = np.ascontiguousarray(fb)
+            rc = np.ascontiguousarray(np.concatenate(colors).astype(np.float32)) if total else \
+                np.zeros((1, 4), np.float32)
+            out = {"rgba8": np.empty((H, W, 4), np.uint8), "depth": np.empty((H, W), np.float32),
+                   "ids": np.empty((H, W), np.uint32), "rgba32": np.empty((H, W, 4), np.float32)}
+            lib.or_resolve(fb.ctypes.data, quad.ctypes.data, rc.ctypes.data, C.byref(view), C.byref(prm),
+                           out["rgba8"].ctypes.data, out["depth"].ctypes.data, out["ids"].ctypes.data,
+                           out["rgba32"].ctypes.data)
+            out["ranges"] = np.concatenate([x[2] for x in all_ranges]) if all_ranges else np.zeros(0, RANGE_DTYPE)
+            out["n_bounds"] = n_bounds
+            results.append(out)
+    return results
