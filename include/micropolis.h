@@ -96,6 +96,9 @@ typedef struct mp_stats {
     int64_t coverage_samples;         /* (micropolygon, pixel) samples written */
     double  backward_ms;              /* device time of the last mp_backward    */
     double  frame_ms;                 /* mp_prepare..mp_finish device time     */
+    int64_t sorted_ranges;            /* ranges of patches with more than 4096 */
+                                      /* DRAW ranges, put in canonical order   */
+                                      /* by a radix sort                       */
 } mp_stats;
 
 typedef struct mp_context mp_context;

@@ -57,6 +57,7 @@ class mp_stats(C.Structure):
         ("max_patches", C.c_int64), ("slow_blocks", C.c_int64),
         ("overflow_relaunches", C.c_int64), ("coverage_samples", C.c_int64),
         ("backward_ms", C.c_double), ("frame_ms", C.c_double),
+        ("sorted_ranges", C.c_int64),
     ]
 
 

@@ -118,6 +118,7 @@ struct mp_context {
 
     // canonical lists
     RangeRec* d_draw = nullptr; RangeRec* d_bucket = nullptr; RangeRec* d_canon = nullptr;
+    unsigned long long* d_sort_keys = nullptr; uint32_t* d_sort_vals = nullptr;   // 2 x list_cap each
     uint32_t list_cap = 0;
     uint32_t* d_counts = nullptr; uint32_t* d_offsets = nullptr; uint32_t* d_cursor = nullptr;
     uint32_t* d_part = nullptr;
@@ -470,6 +471,7 @@ extern "C" void mp_destroy(mp_context* ctx)
     hipFree(ctx->d_level_a); hipFree(ctx->d_level_b); hipFree(ctx->d_level_cnt); hipHostFree(ctx->h_level_cnt);
     hipFree(ctx->d_seg_a); hipFree(ctx->d_seg_b); hipFree(ctx->d_flags); hipFree(ctx->d_blk);
     hipFree(ctx->d_draw); hipFree(ctx->d_bucket); hipFree(ctx->d_canon);
+    hipFree(ctx->d_sort_keys); hipFree(ctx->d_sort_vals);
     hipFree(ctx->d_counts); hipFree(ctx->d_offsets); hipFree(ctx->d_cursor); hipFree(ctx->d_part);
     hipFree(ctx->d_quad);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
@@ -630,7 +632,8 @@ static int ensure_lists(mp_context* ctx, uint32_t cap)
     int rc;
     if ((rc = dev_alloc(ctx, &ctx->d_draw, cap)) || (rc = dev_alloc(ctx, &ctx->d_bucket, cap)) ||
         (rc = dev_alloc(ctx, &ctx->d_canon, cap)) || (rc = dev_alloc(ctx, &ctx->d_keys, cap)) ||
-        (rc = dev_alloc(ctx, &ctx->d_span, cap)))
+        (rc = dev_alloc(ctx, &ctx->d_span, cap)) || (rc = dev_alloc(ctx, &ctx->d_sort_keys, 2 * (size_t)cap)) ||
+        (rc = dev_alloc(ctx, &ctx->d_sort_vals, 2 * (size_t)cap)))
         return rc;
     const size_t q = (size_t)cap * ctx->cfg.reyes_patch_size * ctx->cfg.reyes_patch_size;
     if ((rc = dev_alloc(ctx, &ctx->d_quad, q))) return rc;
@@ -650,7 +653,9 @@ static int grow_lists(mp_context* ctx, uint32_t need, uint32_t keep)
     if ((rc = dev_grow(ctx, &ctx->d_draw, &cap, need, keep))) return rc;
     const uint32_t ncap = (uint32_t)cap;
     if ((rc = dev_alloc(ctx, &ctx->d_bucket, ncap)) || (rc = dev_alloc(ctx, &ctx->d_canon, ncap)) ||
-        (rc = dev_alloc(ctx, &ctx->d_keys, ncap)) || (rc = dev_alloc(ctx, &ctx->d_span, ncap)))
+        (rc = dev_alloc(ctx, &ctx->d_keys, ncap)) || (rc = dev_alloc(ctx, &ctx->d_span, ncap)) ||
+        (rc = dev_alloc(ctx, &ctx->d_sort_keys, 2 * (size_t)ncap)) ||
+        (rc = dev_alloc(ctx, &ctx->d_sort_vals, 2 * (size_t)ncap)))
         return rc;
     const size_t q = (size_t)ncap * ctx->cfg.reyes_patch_size * ctx->cfg.reyes_patch_size;
     if ((rc = dev_alloc(ctx, &ctx->d_quad, q))) return rc;
@@ -690,6 +695,8 @@ static CanonBuffers canon_buffers(mp_context* ctx)
     cb.cursor = ctx->d_cursor;
     cb.part = ctx->d_part;
     cb.n_gpids = ctx->n_gpids;
+    cb.sort_keys = ctx->d_sort_keys;
+    cb.sort_vals = ctx->d_sort_vals;
     return cb;
 }
 
@@ -1178,6 +1185,7 @@ static int execute_frame(mp_context* ctx)
     st.slow_blocks = ctx->h_fc->slow_blocks;
     st.overflow_relaunches = relaunches;
     st.coverage_samples = (int64_t)ctx->h_fc->atomics;
+    st.sorted_ranges = (int64_t)ctx->h_fc->big_ranges;
     st.frame_ms = ms;
     ctx->device_driven = device_driven;
     ctx->frame_valid = true;
@@ -1410,7 +1418,7 @@ static std::vector<std::pair<std::string, size_t>> memory_by_use(const mp_contex
                          ctx->shared_entries * sizeof(RangeRec) + (size_t)ctx->chunk_cap * (64 * sizeof(RangeRec) + 8) +
                          (size_t)ctx->level_cap * 2 * sizeof(RangeRec) + (ctx->seg_a_cap + ctx->seg_b_cap) * sizeof(RangeRec) +
                          ctx->flags_cap + ctx->blk_cap * 4;
-    const size_t lists = (size_t)ctx->list_cap * (3 * sizeof(RangeRec) + 8) + ctx->quad_cap * sizeof(float) +
+    const size_t lists = (size_t)ctx->list_cap * (3 * sizeof(RangeRec) + 8 + 8 + 24) + ctx->quad_cap * sizeof(float) +
                          (size_t)ctx->gpid_cap * 12 + ctx->visible_cap * 4;
     return { { "patch data", patches }, { "framebuffer", fb }, { "bound&split", split }, { "range lists", lists } };
 }

@@ -57,6 +57,10 @@ struct CanonBuffers {
     uint32_t* cursor;        // scatter cursors
     uint32_t* part;          // block sums of the counts (many patches)
     uint32_t n_gpids;
+    // radix-sort lists of buckets too large for the rank count: 2 x cap keys
+    // and 2 x cap slot indices (ping-pong)
+    unsigned long long* sort_keys;
+    uint32_t* sort_vals;
 };
 
 hipError_t launch_reset(FrameCounters* fc, ShareState* share, hipStream_t s);

@@ -43,6 +43,7 @@ MP_DEV void reset_counters(FrameCounters* fc, ShareState* share)
     fc->slow_blocks = 0;
     fc->max_stack = 0;
     fc->total_ranges = 0;
+    fc->big_ranges = 0;
     fc->atomics = 0;
     fc->raster_next = 0;
     fc->uncached_octaves = 0;
@@ -1180,12 +1181,93 @@ __global__ __launch_bounds__(1024) void k_scan_scatter(const DevObject* __restri
 // key).  A work-group takes 256 consecutive bucketed ranges; the keys of
 // every patch they belong to form one contiguous span, staged in LDS once
 // and counted from there (spans above kRankStage keys count from global).
+// A bucket of more than kRankStage ranges is not counted (that would cost
+// its size squared) but sorted by the work-group whose 256 ranges include
+// the bucket's first slot: a stable LSD radix sort of (key, slot) pairs,
+// linear in the bucket size.
 constexpr uint32_t kRankStage = 4096;
 
-__global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, const FrameCounters* __restrict__ fc)
+// Stable LSD radix sort of the n keys at keys[0..n) (48-bit, unique within
+// a bucket) by one 256-thread work-group, 8-bit digits, ping-ponging through
+// ka/va and kb/vb; returns the slot indices in key order in vb.  Within a
+// tile of 256 keys a key's place among the equal digits before it is its rank
+// among its wave's peers (eight ballots) plus the counts of the earlier waves.
+MP_DEV void bucket_radix_sort(const unsigned long long* __restrict__ keys, uint32_t n, unsigned long long* ka,
+                              uint32_t* va, unsigned long long* kb, uint32_t* vb, uint32_t* s_base,
+                              uint32_t (*s_wcnt)[256])
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    constexpr int kPasses = 6;   // 48 key bits
+    for (int pass = 0; pass < kPasses; ++pass) {
+        const int shift = 8 * pass;
+        // pass 0 reads the bucket's keys; the passes alternate a -> b -> a ...
+        // and the sixth writes b
+        const unsigned long long* ksrc = pass == 0 ? keys : ((pass & 1) ? ka : kb);
+        const uint32_t* vsrc = (pass & 1) ? va : vb;
+        unsigned long long* kdst = (pass & 1) ? kb : ka;
+        uint32_t* vdst = (pass & 1) ? vb : va;
+        s_base[tid] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += 256u)
+            atomicAdd(&s_base[(uint32_t)(ksrc[i] >> shift) & 255u], 1u);
+        __syncthreads();
+        {
+            // exclusive scan of the 256 bins (one per thread)
+            const uint32_t x = s_base[tid];
+            uint32_t v = x;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(v, o);
+                if ((int)lane >= o) v += y;
+            }
+            if (lane == 63) s_wcnt[0][wave] = v;
+            __syncthreads();
+            uint32_t before = 0;
+            for (uint32_t w = 0; w < wave; ++w) before += s_wcnt[0][w];
+            __syncthreads();
+            s_base[tid] = before + v - x;
+        }
+        __syncthreads();
+        for (uint32_t t0 = 0; t0 < n; t0 += 256u) {
+            const uint32_t i = t0 + tid;
+            const bool valid = i < n;
+            unsigned long long k = 0;
+            uint32_t v = 0, d = 0;
+            if (valid) {
+                k = ksrc[i];
+                v = pass == 0 ? i : vsrc[i];
+                d = (uint32_t)(k >> shift) & 255u;
+            }
+#pragma unroll
+            for (int w = 0; w < 4; ++w) s_wcnt[w][tid] = 0;
+            __syncthreads();
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint64_t m = __ballot(valid && ((d >> b) & 1u));
+                peers &= ((d >> b) & 1u) ? m : ~m;
+            }
+            if (valid && (uint32_t)(__ffsll((unsigned long long)peers) - 1) == lane)
+                s_wcnt[wave][d] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            if (valid) {
+                uint32_t off = s_base[d] + lanes_below(peers);
+                for (uint32_t w = 0; w < wave; ++w) off += s_wcnt[w][d];
+                kdst[off] = k;
+                vdst[off] = v;
+            }
+            __syncthreads();
+            s_base[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] + s_wcnt[3][tid];
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, FrameCounters* __restrict__ fc)
 {
     __shared__ unsigned long long s_key[kRankStage];
     __shared__ uint32_t s_lo[4], s_hi[4];
+    __shared__ uint32_t s_big_lo, s_big_hi;
     const uint32_t total = fc->total_ranges;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     // every path leaves the per-patch counts and bucket cursors zero for the
@@ -1208,7 +1290,10 @@ __global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, const FrameCounte
             key = cb.keys[j];
             r = cb.bucket[j];
         }
-        uint32_t lo = sp.x, hi = sp.y;
+        // a range of a bucket above kRankStage is placed by the bucket's sort
+        const bool big = valid && sp.y - sp.x > kRankStage;
+        if (tid == 0) s_big_hi = 0u;
+        uint32_t lo = big ? 0xffffffffu : sp.x, hi = big ? 0u : sp.y;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
@@ -1216,13 +1301,14 @@ __global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, const FrameCounte
         }
         if (lane == 0) { s_lo[wave] = lo; s_hi[wave] = hi; }
         __syncthreads();
+        if (big && j == sp.x) { s_big_lo = sp.x; s_big_hi = sp.y; }   // this window holds its first slot
         const uint32_t L = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
         const uint32_t H = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
         const bool staged = H - L <= kRankStage;
         if (staged)
             for (uint32_t k = L + tid; k < H; k += 256u) s_key[k - L] = cb.keys[k];
         __syncthreads();
-        if (valid) {
+        if (valid && !big) {
             uint32_t rank = 0;
             if (staged) {
                 // eight LDS reads in flight per step
@@ -1242,6 +1328,18 @@ __global__ __launch_bounds__(256) void k_rank(CanonBuffers cb, const FrameCounte
             if (sp.x + rank < cb.cap) cb.canon[sp.x + rank] = r;
         }
         __syncthreads();   // s_key / s_lo reused by the next 256 ranges
+        if (s_big_hi) {
+            // sort the big bucket whose first slot lies in this window
+            const uint32_t b0 = s_big_lo, n = s_big_hi - s_big_lo;
+            uint32_t* s_base = reinterpret_cast<uint32_t*>(s_key);
+            uint32_t (*s_wcnt)[256] = reinterpret_cast<uint32_t (*)[256]>(s_base + 256);
+            bucket_radix_sort(cb.keys + b0, n, cb.sort_keys + b0, cb.sort_vals + b0, cb.sort_keys + cb.cap + b0,
+                              cb.sort_vals + cb.cap + b0, s_base, s_wcnt);
+            const uint32_t* order = cb.sort_vals + cb.cap + b0;
+            for (uint32_t i = tid; i < n; i += 256u) cb.canon[b0 + i] = cb.bucket[b0 + order[i]];
+            if (tid == 0) atomicAdd(&fc->big_ranges, n);
+            __syncthreads();
+        }
     }
 }
 

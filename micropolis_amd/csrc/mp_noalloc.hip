@@ -151,9 +151,11 @@ extern "C" int mp_render_noalloc(const mp_config* cfg, const float* xyz, size_t 
     const size_t o_cursor = c.take(((size_t)P + 1) * 4), o_part = c.take(((size_t)P / 4096 + 2) * 4);
     const size_t o_keys = c.take(n_keys * 8);
     // range lists: draw, bucket and canonical records, in-patch keys, spans,
-    // Lambert terms; the rest of the array (less the six regions' alignment)
-    const size_t per_range = 3 * sizeof(RangeRec) + 8 + 8 + 4 * (size_t)N * N;
-    const size_t fixed = c.off + 6 * 256;
+    // Lambert terms; the rest of the array (less the eight regions' alignment)
+    // (+ the radix-sort lists of patches with more than 4096 ranges: 2 keys and
+    // 2 slot indices per range)
+    const size_t per_range = 3 * sizeof(RangeRec) + 8 + 8 + 4 * (size_t)N * N + 2 * 8 + 2 * 4;
+    const size_t fixed = c.off + 8 * 256;
     if (fixed >= kScratchBytes || (kScratchBytes - fixed) / per_range < std::max<size_t>(P, 256))
         return fail_noalloc(MP_E_INVALID,
                             "frame too large for the 64 MiB scratch array (" + std::to_string(P) + " patches, " +
@@ -166,6 +168,8 @@ extern "C" int mp_render_noalloc(const mp_config* cfg, const float* xyz, size_t 
     const size_t o_rkeys = c.take((size_t)cap * 8);
     const size_t o_span = c.take((size_t)cap * 8);
     const size_t o_quad = c.take((size_t)cap * N * N * sizeof(float));
+    const size_t o_skeys = c.take((size_t)cap * 2 * 8);
+    const size_t o_svals = c.take((size_t)cap * 2 * 4);
     if (c.off > kScratchBytes) return fail_noalloc(MP_E_INVALID, "scratch carve exceeds the array");
 
     DevObject* d_obj = reinterpret_cast<DevObject*>(base + o_obj);
@@ -232,6 +236,8 @@ extern "C" int mp_render_noalloc(const mp_config* cfg, const float* xyz, size_t 
     cb.cursor = reinterpret_cast<uint32_t*>(base + o_cursor);
     cb.part = reinterpret_cast<uint32_t*>(base + o_part);
     cb.n_gpids = P;
+    cb.sort_keys = reinterpret_cast<unsigned long long*>(base + o_skeys);
+    cb.sort_vals = reinterpret_cast<uint32_t*>(base + o_svals);
     float* d_quad = reinterpret_cast<float*>(base + o_quad);
 
     // ---- the frame ----

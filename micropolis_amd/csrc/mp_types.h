@@ -88,7 +88,7 @@ struct FrameCounters {
     uint32_t slow_blocks;        // raster blocks on the wide-integer path
     uint32_t max_stack;          // largest LDS stack height seen
     uint32_t total_ranges;       // canonical ranges (written by the scan)
-    uint32_t pad_;
+    uint32_t big_ranges;         // ranges of buckets ordered by the radix sort
     unsigned long long atomics;  // (micropolygon, pixel) coverage events = key atomics
     uint32_t raster_next;        // next canonical range to rasterise
     uint32_t uncached_octaves;   // noise octaves evaluated by hashing

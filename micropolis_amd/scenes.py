@@ -264,4 +264,22 @@ def c4_gregory(seed=4, width=1920, height=1080, planes=32, quads=512):
     return Workload("c4_gregory_%dp_%dx%d_balanced" % (len(pats), width, height), opts, [View(proj, [o])])
 
 
+def big_patch(limit=2.83, width=1920, height=1080):
+    """One bicubic patch, a plane tilted away from the eye (z -0.9 .. -2.2)
+    filling a third of a 90-degree view: LOCAL split with bound_n_split_limit
+    `limit` and N 8 gives 154,634 DRAW ranges of that one patch at 2.83 (split
+    depths 15-19) and 90,654 at 4.0, the case a per-patch rank by counting
+    would pay for with the square of the count."""
+    P = np.zeros((16, 3))
+    for i in range(4):
+        for j in range(4):
+            P[4 * i + j] = (-1.2 + 2.4 * i / 3.0, -0.8 + 1.3 * j / 3.0, -0.9 - 1.3 * j / 3.0)
+    P = orient(P, np.array([0.0, 0.3, 1.0]))[None].astype(np.float32)
+    o = _obj(1, P, BEZIER, np.eye(4), (0.7, 0.7, 0.7, 1.0))
+    opts = dict(window_size=(width, height), bound_n_split_method="LOCAL", bound_n_split_limit=float(limit),
+                reyes_patch_size=8, local_bns_work_groups=0)
+    return Workload("big_patch_%dx%d_limit%g" % (width, height, limit), opts,
+                    [View(Projection(90.0, 0.01, (width, height)), [o])])
+
+
 WORKLOADS = {"c1": c1_sphere, "c2": c2_torus, "c3": c3_batch, "c4": c4_gregory, "c5": c5_views}
