@@ -210,7 +210,8 @@ int mp_cpu_bound_n_split(const mp_config* cfg, const float* xyz, size_t n_points
 
 /* Renderer::dump_trace (src/Reyes/Renderer.h:44): write the stages of the
  * last frame (run once more with events between its stages for LOCAL and
- * BALANCED frames; the whole frame otherwise) in the reference's trace
+ * BALANCED frames; for CPU-method frames each batch's host split and device
+ * raster as recorded; the whole frame otherwise) in the reference's trace
  * format, one line per stage,
  *   name@queue:queued:submit:start:end:id:dependencies
  * with host-clock nanoseconds (src/CL/Device.cpp:275-289). */

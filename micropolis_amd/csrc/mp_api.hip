@@ -170,6 +170,19 @@ struct mp_context {
     // CPU split method
     CpuSplitFrame cpu_frame;
     int cpu_threads = 0;         // 0: the host's cores (at most 16)
+    // its transfer ring (BoundNSplitCLCPU's batch records): bns_pipeline_length
+    // pinned slots of reyes_patches_per_pass ranges; a slot is refilled once
+    // the copy of its previous batch has completed
+    RangeRec* h_ring = nullptr;
+    int ring_len = 0;
+    size_t ring_slot = 0;
+    std::vector<hipEvent_t> ring_ev;
+    // trace of the last CPU-method frame: per batch the host split interval
+    // (ms from the frame's start) and events around its device raster
+    struct BatchTrace { double split0, split1; int n; };
+    std::vector<BatchTrace> cpu_trace;
+    std::vector<hipEvent_t> trace_ev;   // 2 per batch
+    hipEvent_t trace_ev0 = nullptr;
 };
 
 #define HIPCHK(ctx, expr)                                                                  \
@@ -474,6 +487,10 @@ extern "C" void mp_destroy(mp_context* ctx)
     hipFree(ctx->d_sort_keys); hipFree(ctx->d_sort_vals);
     hipFree(ctx->d_counts); hipFree(ctx->d_offsets); hipFree(ctx->d_cursor); hipFree(ctx->d_part);
     hipFree(ctx->d_quad);
+    if (ctx->h_ring) hipHostFree(ctx->h_ring);
+    for (hipEvent_t e : ctx->ring_ev) hipEventDestroy(e);
+    for (hipEvent_t e : ctx->trace_ev) hipEventDestroy(e);
+    if (ctx->trace_ev0) hipEventDestroy(ctx->trace_ev0);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -1038,9 +1055,53 @@ static int run_segment_split(mp_context* ctx, uint32_t& draw_total, int64_t& dro
     return MP_OK;
 }
 
-// CPU method: host depth-first split on the host cores, DRAW list and
-// per-patch counts uploaded for the device back end
-static int run_cpu_split(mp_context* ctx, uint32_t& draw_total, int64_t& max_patches)
+// Grows the canonical list (and the Lambert store) to hold `need` ranges,
+// keeping the first `keep` of both; the other lists are reallocated empty.
+static int grow_canon(mp_context* ctx, uint32_t need, uint32_t keep)
+{
+    if (need <= ctx->list_cap) return MP_OK;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    size_t cap = ctx->list_cap;
+    int rc;
+    if ((rc = dev_grow(ctx, &ctx->d_canon, &cap, need, keep))) return rc;
+    const size_t N2 = (size_t)ctx->cfg.reyes_patch_size * ctx->cfg.reyes_patch_size;
+    float* quad = nullptr;
+    if (hipMalloc((void**)&quad, cap * N2 * sizeof(float)) != hipSuccess)
+        return fail(ctx, MP_E_NOMEM, "hipMalloc: Lambert store");
+    if (keep) HIPCHK(ctx, hipMemcpy(quad, ctx->d_quad, (size_t)keep * N2 * sizeof(float), hipMemcpyDeviceToDevice));
+    hipFree(ctx->d_quad);
+    ctx->d_quad = quad;
+    ctx->quad_cap = cap * N2;
+    const uint32_t ncap = (uint32_t)cap;
+    if ((rc = dev_alloc(ctx, &ctx->d_draw, ncap)) || (rc = dev_alloc(ctx, &ctx->d_bucket, ncap)) ||
+        (rc = dev_alloc(ctx, &ctx->d_keys, ncap)) || (rc = dev_alloc(ctx, &ctx->d_span, ncap)) ||
+        (rc = dev_alloc(ctx, &ctx->d_sort_keys, 2 * (size_t)ncap)) ||
+        (rc = dev_alloc(ctx, &ctx->d_sort_vals, 2 * (size_t)ncap)))
+        return rc;
+    ctx->list_cap = ncap;
+    invalidate_graph(ctx);
+    return MP_OK;
+}
+
+static int hip_event(mp_context* ctx, std::vector<hipEvent_t>& pool, size_t k)
+{
+    while (pool.size() <= k) {
+        hipEvent_t e = nullptr;
+        HIPCHK(ctx, hipEventCreate(&e));
+        pool.push_back(e);
+    }
+    return MP_OK;
+}
+
+// CPU method (BoundNSplitCLCPU.cpp:75-228): the host splits the patches on
+// its cores and hands the DRAW ranges over in canonical order (patches in
+// order, each patch's ranges sorted on the host), so every batch of
+// reyes_patches_per_pass ranges goes straight into the canonical list at its
+// final place: it is copied from a slot of the pinned ring of
+// bns_pipeline_length buffers and rasterised by a launch of its own while the
+// host splits the next patches.  The ring slot is reused once its copy has
+// completed.  The resolve follows the last batch.
+static int run_cpu_frame(mp_context* ctx, uint32_t& draw_total, int64_t& max_patches)
 {
     const int no = (int)ctx->objects.size();
     std::vector<CpuSplitObject> objs(no);
@@ -1062,18 +1123,89 @@ static int run_cpu_split(mp_context* ctx, uint32_t& draw_total, int64_t& max_pat
         o.gpid_base = d.gpid_base;
         o.slot = (uint32_t)oi;
     }
-    CpuSplitFrame& fr = ctx->cpu_frame;
-    cpu_split_frame(objs, ctx->n_gpids, ctx->cfg.cull_ribbon, ctx->cfg.max_split_depth,
-                    ctx->cfg.reyes_patches_per_pass, ctx->cpu_threads, fr);
-    draw_total = (uint32_t)fr.draws.size();
+    const size_t slot_cap = (size_t)std::max(1, ctx->cfg.reyes_patches_per_pass);
+    const int ring_len = std::max(1, ctx->cfg.bns_pipeline_length);
+    if (!ctx->h_ring || ctx->ring_len != ring_len || ctx->ring_slot != slot_cap) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->h_ring) hipHostFree(ctx->h_ring);
+        ctx->h_ring = nullptr;
+        HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_ring, (size_t)ring_len * slot_cap * sizeof(RangeRec), 0));
+        ctx->ring_len = ring_len;
+        ctx->ring_slot = slot_cap;
+    }
     int rc;
-    if ((rc = grow_lists(ctx, std::max<uint32_t>(draw_total, 1), 0))) return rc;
-    if (draw_total)
-        HIPCHK(ctx, hipMemcpyAsync(ctx->d_draw, fr.draws.data(), (size_t)draw_total * sizeof(RangeRec),
-                                   hipMemcpyHostToDevice, ctx->stream));
-    if (ctx->n_gpids)
-        HIPCHK(ctx, hipMemcpyAsync(ctx->d_counts, fr.counts.data(), (size_t)ctx->n_gpids * sizeof(uint32_t),
-                                   hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = hip_event(ctx, ctx->ring_ev, (size_t)ring_len - 1))) return rc;
+    if (!ctx->trace_ev0) HIPCHK(ctx, hipEventCreate(&ctx->trace_ev0));
+    std::vector<bool> busy((size_t)ring_len, false);
+    DevParams prm = ctx->prm;
+    prm.raster_chunk = 0;   // windows take the static range schedule
+    const bool render = !ctx->cfg.dummy_render;
+
+    ctx->cpu_trace.clear();
+    const auto t0 = std::chrono::steady_clock::now();
+    auto now_ms = [&] {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    HIPCHK(ctx, hipEventRecord(ctx->trace_ev0, ctx->stream));
+    uint32_t shipped = 0;
+    size_t fill = 0;
+    int slot = 0, cur_view = -1;
+    double batch_t0 = 0.0;
+    int err = MP_OK;
+    auto ship = [&]() -> int {
+        if (!fill) return MP_OK;
+        int r2;
+        if ((r2 = grow_canon(ctx, shipped + (uint32_t)fill, shipped))) return r2;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_canon + shipped, ctx->h_ring + (size_t)slot * slot_cap,
+                                   fill * sizeof(RangeRec), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->ring_ev[slot], ctx->stream));
+        busy[slot] = true;
+        const size_t b = ctx->cpu_trace.size();
+        if ((r2 = hip_event(ctx, ctx->trace_ev, 2 * b + 1))) return r2;
+        ctx->cpu_trace.push_back(mp_context::BatchTrace{ batch_t0, now_ms(), (int)fill });
+        HIPCHK(ctx, hipEventRecord(ctx->trace_ev[2 * b], ctx->stream));
+        if (render)
+            HIPCHK(ctx, launch_raster(ctx->cfg.reyes_patch_size, ctx->d_canon, ctx->d_objs, ctx->d_views, prm,
+                                      ctx->d_fc, ctx->d_quad, ctx->raster_grid_size, ctx->stream, shipped,
+                                      (uint32_t)fill));
+        HIPCHK(ctx, hipEventRecord(ctx->trace_ev[2 * b + 1], ctx->stream));
+        shipped += (uint32_t)fill;
+        fill = 0;
+        slot = (slot + 1) % ring_len;
+        if (busy[slot]) HIPCHK(ctx, hipEventSynchronize(ctx->ring_ev[slot]));   // its copy has read the slot
+        busy[slot] = false;
+        batch_t0 = now_ms();
+        return MP_OK;
+    };
+    auto emit = [&](const RangeRec* r, size_t n) {
+        for (size_t i = 0; i < n && err == MP_OK; ++i) {
+            const int v = ctx->h_objs[r[i].obj_depth & 0xffffffu].view;
+            while (cur_view < v && err == MP_OK) {
+                ++cur_view;   // the view's first range: its canonical base
+                if (launch_set_view_base(ctx->d_views, cur_view, shipped + (uint32_t)fill, nullptr, 0u,
+                                         ctx->stream) != hipSuccess)
+                    err = fail(ctx, MP_E_HIP, "launch_set_view_base failed");
+            }
+            ctx->h_ring[(size_t)slot * slot_cap + fill++] = r[i];
+            if (fill == slot_cap && err == MP_OK) err = ship();
+        }
+    };
+    CpuSplitFrame& fr = ctx->cpu_frame;
+    cpu_split_stream(objs, ctx->n_gpids, ctx->cfg.cull_ribbon, ctx->cfg.max_split_depth,
+                     ctx->cfg.reyes_patches_per_pass, ctx->cpu_threads, fr, emit);
+    if (err) return err;
+    if ((rc = ship())) return rc;
+    // views with no range, and the frame's range total
+    while (cur_view + 1 < ctx->n_views) {
+        ++cur_view;
+        HIPCHK(ctx, launch_set_view_base(ctx->d_views, cur_view, shipped, nullptr, 0u, ctx->stream));
+    }
+    HIPCHK(ctx, launch_set_view_base(ctx->d_views, -1, 0u, ctx->d_fc, shipped, ctx->stream));
+    if (render)
+        HIPCHK(ctx, launch_resolve(ctx->d_views, ctx->n_views, ctx->max_pixels, ctx->d_canon, ctx->d_objs,
+                                   ctx->d_quad, ctx->cfg.reyes_patch_size, ctx->cfg.write_rgba32,
+                                   ctx->d_display, ctx->stream));
+    draw_total = shipped;
     ctx->pass_counts = fr.pass_counts;
     max_patches = fr.max_stack;
     return MP_OK;
@@ -1151,8 +1283,7 @@ static int execute_frame(mp_context* ctx)
         if (ctx->split_engine == 1 && (rc = update_level_launches(ctx, &max_patches))) return rc;
     } else if (method == MP_BNS_CPU) {
         HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
-        if ((rc = run_cpu_split(ctx, draw_total, max_patches))) return rc;
-        if ((rc = launch_back_end(ctx))) return rc;
+        if ((rc = run_cpu_frame(ctx, draw_total, max_patches))) return rc;
         HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     } else {
@@ -1230,12 +1361,7 @@ extern "C" int mp_cpu_bound_n_split(const mp_config* cfg, const float* xyz, size
     const char* e = getenv("MP_CPU_THREADS");
     cpu_split_frame(std::vector<CpuSplitObject>(1, o), (uint32_t)o.n_patches, cfg->cull_ribbon,
                     cfg->max_split_depth, cfg->reyes_patches_per_pass, e ? atoi(e) : 0, fr);
-    // canonical order: (pid, umin, vmin); umin, vmin are dyadic, so compare as floats
-    std::sort(fr.draws.begin(), fr.draws.end(), [](const RangeRec& a, const RangeRec& b) {
-        if (a.pid != b.pid) return a.pid < b.pid;
-        if (a.umin != b.umin) return a.umin < b.umin;
-        return a.vmin < b.vmin;
-    });
+    // fr.draws is in canonical order: (pid, umin, vmin)
     const int64_t n = (int64_t)fr.draws.size();
     if (count) *count = n;
     if (records) {
@@ -1480,6 +1606,25 @@ extern "C" int mp_dump_trace(mp_context* ctx, const char* path)
         for (auto& e : ev)
             if (e) hipEventDestroy(e);
         if (rc) return rc;
+    } else if (ctx->cfg.bound_n_split_method == MP_BNS_CPU && !ctx->cpu_trace.empty()) {
+        // per batch: the host split that filled it (host clock) and its device
+        // raster (HIP events, from the frame's first event, which the stream
+        // reaches right after the host clock starts); a batch's raster depends
+        // on its split, the next split runs meanwhile
+        std::vector<std::string> lines;
+        for (size_t k = 0; k < ctx->cpu_trace.size(); ++k) {
+            float a = 0.0f, b = 0.0f;
+            if (hipEventElapsedTime(&a, ctx->trace_ev0, ctx->trace_ev[2 * k]) != hipSuccess ||
+                hipEventElapsedTime(&b, ctx->trace_ev0, ctx->trace_ev[2 * k + 1]) != hipSuccess)
+                return fail(ctx, MP_E_HIP, "trace events");
+            const uint64_t h0 = base + (uint64_t)(ctx->cpu_trace[k].split0 * 1e6);
+            const uint64_t h1 = base + (uint64_t)(ctx->cpu_trace[k].split1 * 1e6);
+            const uint64_t d0 = base + (uint64_t)((double)a * 1e6), d1 = base + (uint64_t)((double)b * 1e6);
+            fs << "bound & split@host:" << h0 << ":" << h0 << ":" << h0 << ":" << h1 << ":" << 2 * k << ":\n";
+            fs << "sample@device:" << h1 << ":" << h1 << ":" << d0 << ":" << d1 << ":" << 2 * k + 1 << ":" << 2 * k
+               << "\n";
+        }
+        return fs.good() ? MP_OK : fail(ctx, MP_E_INVALID, "write failed");
     } else {
         recs.push_back(Rec{ "frame", 0.0, ctx->stats.frame_ms });
     }
@@ -1508,13 +1653,12 @@ extern "C" int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* 
             uint32_t dt = 0;
             HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
             if (ctx->cfg.bound_n_split_method == MP_BNS_CPU) {
-                HIPCHK(ctx, hipStreamSynchronize(ctx->stream));   // the host lists are reused
-                rc = run_cpu_split(ctx, dt, maxp);
+                rc = run_cpu_frame(ctx, dt, maxp);   // split, ring, rasters and resolve
             } else {
                 rc = run_segment_split(ctx, dt, dropped, maxp);
+                if (!rc) rc = launch_back_end(ctx);
             }
             if (rc) return rc;
-            if ((rc = launch_back_end(ctx))) return rc;
         }
     } else if (use_graph) {
         if (!ctx->graph) {

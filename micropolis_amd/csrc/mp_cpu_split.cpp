@@ -14,6 +14,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
 #include <thread>
 
 namespace mp {
@@ -142,6 +145,7 @@ void split_roots(const CpuSplitObject& o, int p0, int p1, float ribbon, int msd,
     for (int pid = p0; pid < p1; ++pid) {
         const float* cp = o.cps + (size_t)pid * per * 4;
         RootStats rs{ 0, 1, 0 };
+        const size_t first = out.size();
         st.clear();
         st.push_back(RangeRec{ (uint32_t)pid, o.slot, 0.0f, 0.0f, 1.0f, 1.0f });
         while (!st.empty()) {
@@ -169,15 +173,20 @@ void split_roots(const CpuSplitObject& o, int p0, int p1, float ribbon, int msd,
                 rs.max_local = std::max<uint32_t>(rs.max_local, (uint32_t)st.size());
             }
         }
+        // the patch's DRAW ranges in canonical order: (umin, vmin), dyadic
+        // values compared as floats
+        std::sort(out.begin() + (std::ptrdiff_t)first, out.end(), [](const RangeRec& a, const RangeRec& b) {
+            return a.umin != b.umin ? a.umin < b.umin : a.vmin < b.vmin;
+        });
         stats[pid] = rs;
     }
 }
 
 }  // namespace
 
-void cpu_split_frame(const std::vector<CpuSplitObject>& objs, uint32_t n_gpids,
-                     float cull_ribbon, int max_split_depth, int patches_per_pass,
-                     int n_threads, CpuSplitFrame& out)
+void cpu_split_stream(const std::vector<CpuSplitObject>& objs, uint32_t n_gpids, float cull_ribbon,
+                      int max_split_depth, int patches_per_pass, int n_threads, CpuSplitFrame& out,
+                      const std::function<void(const RangeRec*, size_t)>& emit)
 {
     out.draws.clear();
     out.counts.assign(n_gpids, 0u);
@@ -194,34 +203,49 @@ void cpu_split_frame(const std::vector<CpuSplitObject>& objs, uint32_t n_gpids,
     if (n_threads <= 0) n_threads = std::max(1, std::min((int)std::thread::hardware_concurrency(), 16));
     n_threads = std::max(1, std::min<int>(n_threads, (int)chunks.size()));
 
-    std::vector<std::vector<RangeRec>> part(n_threads);
-    std::vector<int64_t> part_bounds(n_threads, 0);
+    // chunk c's DRAW ranges, ready once done[c] is set
+    std::vector<std::vector<RangeRec>> part(chunks.size());
+    std::vector<int64_t> chunk_bounds(chunks.size(), 0);
+    std::unique_ptr<std::atomic<int>[]> done(new std::atomic<int>[chunks.size() + 1]);
+    for (size_t c = 0; c < chunks.size(); ++c) done[c].store(0);
+    std::mutex m;
+    std::condition_variable cv;
     std::atomic<size_t> next{ 0 };
-    auto work = [&](int t) {
-        for (;;) {
-            const size_t c = next.fetch_add(1, std::memory_order_relaxed);
-            if (c >= chunks.size()) break;
-            const Chunk& ck = chunks[c];
-            const CpuSplitObject& o = objs[ck.obj];
-            split_roots(o, ck.p0, ck.p1, cull_ribbon, max_split_depth, part[t],
-                        stats.data() + o.gpid_base, part_bounds[t]);
+    auto split_chunk = [&](size_t c) {
+        const Chunk& ck = chunks[c];
+        const CpuSplitObject& o = objs[ck.obj];
+        split_roots(o, ck.p0, ck.p1, cull_ribbon, max_split_depth, part[c], stats.data() + o.gpid_base,
+                    chunk_bounds[c]);
+        {
+            std::lock_guard<std::mutex> lk(m);
+            done[c].store(1, std::memory_order_release);
         }
+        cv.notify_all();
     };
-    if (n_threads == 1) {
-        work(0);
-    } else {
-        std::vector<std::thread> pool;
-        for (int t = 1; t < n_threads; ++t) pool.emplace_back(work, t);
-        work(0);
-        for (auto& th : pool) th.join();
+    std::vector<std::thread> pool;
+    if (n_threads > 1) {
+        for (int t = 0; t < n_threads; ++t)
+            pool.emplace_back([&] {
+                for (;;) {
+                    const size_t c = next.fetch_add(1, std::memory_order_relaxed);
+                    if (c >= chunks.size()) break;
+                    split_chunk(c);
+                }
+            });
     }
-    size_t total = 0;
-    for (int t = 0; t < n_threads; ++t) {
-        total += part[t].size();
-        out.bounds += part_bounds[t];
+    // chunks in order: object slots ascending, patches ascending
+    for (size_t c = 0; c < chunks.size(); ++c) {
+        if (n_threads > 1) {
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return done[c].load(std::memory_order_acquire) != 0; });
+        } else {
+            split_chunk(c);
+        }
+        if (!part[c].empty()) emit(part[c].data(), part[c].size());
+        out.bounds += chunk_bounds[c];
+        std::vector<RangeRec>().swap(part[c]);
     }
-    out.draws.reserve(total);
-    for (int t = 0; t < n_threads; ++t) out.draws.insert(out.draws.end(), part[t].begin(), part[t].end());
+    for (auto& th : pool) th.join();
 
     // Per-pass counts and stack height of the reference's one stack: roots are
     // popped from the back, so while root p is processed the roots 0..p-1 are
@@ -245,6 +269,16 @@ void cpu_split_frame(const std::vector<CpuSplitObject>& objs, uint32_t n_gpids,
             last < 0 || last > 0 || stats[o.gpid_base + last].pending;
         if (rem > 0 || draws == 0 || pending_after_last) out.pass_counts.push_back((int32_t)rem);
     }
+}
+
+void cpu_split_frame(const std::vector<CpuSplitObject>& objs, uint32_t n_gpids,
+                     float cull_ribbon, int max_split_depth, int patches_per_pass,
+                     int n_threads, CpuSplitFrame& out)
+{
+    std::vector<RangeRec> all;
+    cpu_split_stream(objs, n_gpids, cull_ribbon, max_split_depth, patches_per_pass, n_threads, out,
+                     [&](const RangeRec* r, size_t n) { all.insert(all.end(), r, r + n); });
+    out.draws.swap(all);
 }
 
 }  // namespace mp

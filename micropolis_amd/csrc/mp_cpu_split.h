@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <vector>
 
 #include "mp_types.h"
@@ -38,9 +39,20 @@ struct CpuSplitFrame {
 };
 
 // Splits all objects.  n_threads <= 0 picks the host's core count (at most 16).
+// out.draws receives every DRAW range in canonical order (object slot, patch,
+// umin, vmin).
 void cpu_split_frame(const std::vector<CpuSplitObject>& objs, uint32_t n_gpids,
                      float cull_ribbon, int max_split_depth, int patches_per_pass,
                      int n_threads, CpuSplitFrame& out);
+
+// The same split, streamed: chunks of patches are split by n_threads worker
+// threads while the calling thread hands the finished chunks, in canonical
+// order (object slot, patch, umin, vmin), to `emit`.  out.draws stays empty;
+// counts, pass counts, bounds and stack height are filled as by
+// cpu_split_frame.
+void cpu_split_stream(const std::vector<CpuSplitObject>& objs, uint32_t n_gpids, float cull_ribbon,
+                      int max_split_depth, int patches_per_pass, int n_threads, CpuSplitFrame& out,
+                      const std::function<void(const RangeRec*, size_t)>& emit);
 
 // One range (exposed for tests of the library's host code): returns MP_CULL,
 // MP_DRAW, MP_CPU_DROP, MP_HSPLIT (halve v) or MP_VSPLIT (halve u).

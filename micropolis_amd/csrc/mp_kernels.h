@@ -94,9 +94,14 @@ uint32_t segment_blocks(uint32_t n);
 hipError_t launch_canonicalize(const DevObject* objs, DevView* views, int n_views,
                                const CanonBuffers& cb, FrameCounters* fc, int num_cus, int fused,
                                hipStream_t s);
+// count == 0: every range of the frame (fc->total_ranges); otherwise the
+// canonical ranges [first, first + count) with the static range schedule
 hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject* objs,
                          const DevView* views, const DevParams& prm, FrameCounters* fc,
-                         float* quad_store, int grid, hipStream_t s);
+                         float* quad_store, int grid, hipStream_t s, uint32_t first = 0, uint32_t count = 0);
+// sets views[view].range_base (view >= 0) and, when fc is given, fc->total_ranges
+hipError_t launch_set_view_base(DevView* views, int view, uint32_t base, FrameCounters* fc, uint32_t total,
+                                hipStream_t s);
 int raster_grid(int patch_size, int num_cus, const DevParams& prm);
 hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           const RangeRec* canon, const DevObject* objs, const float* quad_store,

@@ -1636,7 +1636,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                   const DevObject* __restrict__ objs,
                                                   const DevView* __restrict__ views,
                                                   DevParams prm, FrameCounters* fc,
-                                                  float* __restrict__ quad_store)
+                                                  float* __restrict__ quad_store, uint32_t first,
+                                                  uint32_t count)
 {
     constexpr int G1 = N + 1;
     constexpr int G = G1 * G1;
@@ -1657,7 +1658,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const uint32_t total = fc->total_ranges;
+    // the canonical ranges [first, first + count), or all of the frame's (count 0)
+    const uint32_t total = count ? first + count : fc->total_ranges;
     uint32_t slow = 0, natom = 0, uncached = 0;
     __shared__ uint32_t s_next;
 
@@ -1667,7 +1669,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // returning atomic on one address serialises across the device, so the
     // chunk must stay large against the range count); raster_chunk < 0
     // sizes it from the range count, > 0 fixes it.
-    const bool dyn = prm.raster_chunk != 0;
+    const bool dyn = prm.raster_chunk != 0 && count == 0;
     const uint32_t chunk = prm.raster_chunk > 0 ? (uint32_t)prm.raster_chunk
                                                 : max(8u, min(64u, total / (gridDim.x * 4u)));
     uint32_t r, r_end;
@@ -1677,7 +1679,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         r = s_next;
         r_end = min(r + chunk, total);
     } else {
-        r = blockIdx.x;
+        r = first + blockIdx.x;
         r_end = total;
     }
     if (r < total) {
@@ -1878,30 +1880,53 @@ int raster_grid(int patch_size, int num_cus, const DevParams& prm)
 template <int N, int NT>
 static void launch_raster_n(int mode, int grid, hipStream_t s, const RangeRec* canon,
                             const DevObject* objs, const DevView* views, const DevParams& prm,
-                            FrameCounters* fc, float* quad_store)
+                            FrameCounters* fc, float* quad_store, uint32_t first, uint32_t count)
 {
     if (mode == 0)
-        hipLaunchKernelGGL((k_raster<N, 0, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store);
+        hipLaunchKernelGGL((k_raster<N, 0, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store,
+                           first, count);
     else if (mode == 1)
-        hipLaunchKernelGGL((k_raster<N, 1, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store);
+        hipLaunchKernelGGL((k_raster<N, 1, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store,
+                           first, count);
     else
-        hipLaunchKernelGGL((k_raster<N, 2, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store);
+        hipLaunchKernelGGL((k_raster<N, 2, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store,
+                           first, count);
 }
 
 hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject* objs,
                          const DevView* views, const DevParams& prm, FrameCounters* fc,
-                         float* quad_store, int grid, hipStream_t s)
+                         float* quad_store, int grid, hipStream_t s, uint32_t first, uint32_t count)
 {
     const int mode = raster_mode(prm);
+    if (count) grid = (int)std::min<uint32_t>((uint32_t)grid, count);   // a window needs no more groups than ranges
     switch (patch_size) {
-    case 8: launch_raster_n<8, 64>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
+    case 8: launch_raster_n<8, 64>(mode, grid, s, canon, objs, views, prm, fc, quad_store, first, count); break;
     case 16:
-        if (prm.raster_one_wave) launch_raster_n<16, 64>(mode, grid, s, canon, objs, views, prm, fc, quad_store);
-        else launch_raster_n<16, 256>(mode, grid, s, canon, objs, views, prm, fc, quad_store);
+        if (prm.raster_one_wave)
+            launch_raster_n<16, 64>(mode, grid, s, canon, objs, views, prm, fc, quad_store, first, count);
+        else
+            launch_raster_n<16, 256>(mode, grid, s, canon, objs, views, prm, fc, quad_store, first, count);
         break;
-    case 32: launch_raster_n<32, 1024>(mode, grid, s, canon, objs, views, prm, fc, quad_store); break;
+    case 32: launch_raster_n<32, 1024>(mode, grid, s, canon, objs, views, prm, fc, quad_store, first, count); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// The canonical range base of one view, and the frame's range total (the
+// CPU split method's host order replaces the device ordering stage).
+__global__ void k_set_view_base(DevView* views, int view, uint32_t base, FrameCounters* fc, uint32_t total)
+{
+    if (threadIdx.x == 0) {
+        if (view >= 0) views[view].range_base = base;
+        if (fc) fc->total_ranges = total;
+    }
+}
+
+hipError_t launch_set_view_base(DevView* views, int view, uint32_t base, FrameCounters* fc, uint32_t total,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_view_base, dim3(1), dim3(64), 0, s, views, view, base, fc, total);
     return hipGetLastError();
 }
 
