@@ -2,9 +2,14 @@
 //
 // Every floating-point expression follows the operation order written in the
 // reference kernels (kernels/utility.h, kernels/bound_n_split.h,
-// kernels/dice.cl, kernels/reyes.cl); the library is built with
-// -ffp-contract=off and the default correctly rounded f32 '/' and sqrt, so
-// results are bit-identical to an IEEE-strict scalar evaluation.
+// kernels/dice.cl, kernels/reyes.cl).  The reference builds its kernels with
+// -cl-mad-enable (src/CL/Program.cpp:118), which lets a product and the sum
+// it feeds be contracted; here that contraction is one fixed rule: s + p*q
+// is fma(p, q, s) and p*q + r*s (or -) is fma(r, s, p*q), the left product
+// rounded, each written out with mad() / mad_diff().  Nothing else is
+// contracted (the library is built with -ffp-contract=off) and '/' and sqrt
+// are the default correctly rounded ones, so every result equals the scalar
+// evaluation with C99 fmaf at the same places.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -62,9 +67,14 @@ MP_DEV int32_t f2i_sat(float x)
     return (int32_t)x;
 }
 
+// s + p*q as one fused multiply-add (the contraction rule above)
+MP_DEV float mad(float p, float q, float s) { return __builtin_fmaf(p, q, s); }
+// p*q - r*s: the left product rounded, the right one fused
+MP_DEV float mad_diff(float p, float q, float r, float s) { return __builtin_fmaf(-r, s, p * q); }
+
 MP_DEV float dot3(float ax, float ay, float az, float bx, float by, float bz)
 {
-    return (ax * bx + ay * by) + az * bz;
+    return mad(az, bz, mad(ay, by, ax * bx));
 }
 
 // column-major 4x4 times vector, rows summed left to right
@@ -72,7 +82,29 @@ MP_DEV void mat4_mul(const float* __restrict__ m, const float v[4], float o[4])
 {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        o[i] = ((m[0 + i] * v[0] + m[4 + i] * v[1]) + m[8 + i] * v[2]) + m[12 + i] * v[3];
+        o[i] = mad(m[12 + i], v[3], mad(m[8 + i], v[2], mad(m[4 + i], v[1], m[0 + i] * v[0])));
+}
+
+// proj * e for the perspective matrices of every view (glm::perspective,
+// Projection.cpp:39: only entries 0, 5, 10, 11 = -1 and 14 are non-zero).
+// With e finite and e.z, e.w non-zero the full product's rows reduce to
+// p0 e0, p5 e1, fma(p14, e3, p10 e2) and -e2 exactly (a fused zero product
+// adds a signed zero to a non-zero sum), up to the sign of a zero x
+// or y row (zero terms add signed zeros), which no pixel coordinate
+// computed from q.x / q.w or q.y / q.w can tell apart; the wave takes the
+// full product when any lane's point is not such a point.
+MP_DEV void perspective_mul(const float* __restrict__ m, const float e[4], float q[4])
+{
+    const bool plain = fabsf(e[0]) < INFINITY && fabsf(e[1]) < INFINITY && fabsf(e[2]) < INFINITY &&
+                       fabsf(e[3]) < INFINITY && e[2] != 0.0f && e[3] != 0.0f;
+    if (__all(plain)) {
+        q[0] = m[0] * e[0];
+        q[1] = m[5] * e[1];
+        q[2] = mad(m[14], e[3], m[10] * e[2]);
+        q[3] = -e[2];
+    } else {
+        mat4_mul(m, e, q);
+    }
 }
 
 MP_DEV void bern_u(float t, float s, float a[4])
@@ -99,10 +131,10 @@ MP_DEV void dbern(float t, float s, float d[4])
 
 MP_DEV void acc4(float p[4], float w, const float4 P)
 {
-    p[0] = p[0] + w * P.x;
-    p[1] = p[1] + w * P.y;
-    p[2] = p[2] + w * P.z;
-    p[3] = p[3] + w * P.w;
+    p[0] = mad(w, P.x, p[0]);
+    p[1] = mad(w, P.y, p[1]);
+    p[2] = mad(w, P.z, p[2]);
+    p[3] = mad(w, P.w, p[3]);
 }
 
 // Bicubic Bezier (utility.h:41-72); cp = 16 control points (xyz, w=1).
@@ -134,14 +166,14 @@ MP_DEV void eval_gregory(const float4* __restrict__ P, float u, float v, float p
     const float d0 = u + v, d1 = (1.0f - u) + v, d2 = (2.0f - u) - v, d3 = (1.0f + u) - v;
     const float4 a12 = P[12], a16 = P[16], a17 = P[17], a13 = P[13];
     const float4 a14 = P[14], a18 = P[18], a19 = P[19], a15 = P[15];
-    float4 F0 = greg_f(u * a12.x + v * a16.x, u * a12.y + v * a16.y, u * a12.z + v * a16.z,
-                       u * a12.w + v * a16.w, d0);
-    float4 F1 = greg_f(omu * a17.x + v * a13.x, omu * a17.y + v * a13.y, omu * a17.z + v * a13.z,
-                       omu * a17.w + v * a13.w, d1);
-    float4 F2 = greg_f(omu * a14.x + omv * a18.x, omu * a14.y + omv * a18.y,
-                       omu * a14.z + omv * a18.z, omu * a14.w + omv * a18.w, d2);
-    float4 F3 = greg_f(u * a19.x + omv * a15.x, u * a19.y + omv * a15.y, u * a19.z + omv * a15.z,
-                       u * a19.w + omv * a15.w, d3);
+    float4 F0 = greg_f(mad(v, a16.x, u * a12.x), mad(v, a16.y, u * a12.y), mad(v, a16.z, u * a12.z),
+                       mad(v, a16.w, u * a12.w), d0);
+    float4 F1 = greg_f(mad(v, a13.x, omu * a17.x), mad(v, a13.y, omu * a17.y), mad(v, a13.z, omu * a17.z),
+                       mad(v, a13.w, omu * a17.w), d1);
+    float4 F2 = greg_f(mad(omv, a18.x, omu * a14.x), mad(omv, a18.y, omu * a14.y),
+                       mad(omv, a18.z, omu * a14.z), mad(omv, a18.w, omu * a14.w), d2);
+    float4 F3 = greg_f(mad(omv, a15.x, u * a19.x), mad(omv, a15.y, u * a19.y), mad(omv, a15.z, u * a19.z),
+                       mad(omv, a15.w, u * a19.w), d3);
     float su = 1.0f - u, sv = 1.0f - v;
     float a[4], b[4];
     bern_u(u, su, a);
@@ -195,7 +227,7 @@ MP_DEV float lattice_wave(int32_t x, int32_t y, int32_t z, uint32_t seed)
     return one_plus - ((m >> 23) ? 1.0f : 2.0f);
 }
 MP_DEV float fade(float t) { return (t * t) * (3.0f - 2.0f * t); }
-MP_DEV float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
+MP_DEV float lerpf_(float a, float b, float t) { return mad(b - a, t, a); }
 
 // Value of v held by lane byte_addr / 4 of the wave (ds_bpermute_b32).
 MP_DEV float lane_value(int byte_addr, float v)
@@ -337,15 +369,15 @@ MP_DEV float noise3_kept(const DevParams& prm, float x, float y, float z, NoiseW
 #pragma unroll
     for (int k = 0; k < kKeptOctaves; ++k) {
         if (k < prm.disp_octaves) {
-            acc = acc + amp * vnoise_kept(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, w.bx[k],
-                                          w.by[k], w.bz[k], w.hv[k], w.valid[k]);
+            acc = mad(amp, vnoise_kept(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, w.bx[k],
+                                          w.by[k], w.bz[k], w.hv[k], w.valid[k]), acc);
             norm = norm + amp;
             amp = amp * 0.5f;
             freq = freq * 2.0f;
         }
     }
     for (int k = kKeptOctaves; k < prm.disp_octaves; ++k) {
-        acc = acc + amp * vnoise(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, kNoiseLaneWindow);
+        acc = mad(amp, vnoise(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, kNoiseLaneWindow), acc);
         norm = norm + amp;
         amp = amp * 0.5f;
         freq = freq * 2.0f;
@@ -361,7 +393,7 @@ MP_DEV float noise3(const DevParams& prm, float x, float y, float z, bool full_w
     const int scheme = prm.noise_wave_cells == kNoiseLaneWindow && !full_wave ? kNoiseWaveCell
                                                                               : prm.noise_wave_cells;
     for (int k = 0; k < prm.disp_octaves; ++k) {
-        acc = acc + amp * vnoise(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, scheme);
+        acc = mad(amp, vnoise(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, scheme), acc);
         norm = norm + amp;
         amp = amp * 0.5f;
         freq = freq * 2.0f;
@@ -383,7 +415,7 @@ MP_DEV float4 normal_column(const float4* __restrict__ cp, float u, int k)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const float4 P = cp[4 * i + j];
-        x = x + w[i] * P.x; y = y + w[i] * P.y; z = z + w[i] * P.z;
+        x = mad(w[i], P.x, x); y = mad(w[i], P.y, y); z = mad(w[i], P.z, z);
     }
     return make_float4(x, y, z, 0.0f);
 }
@@ -401,18 +433,66 @@ MP_DEV void normal_from_columns(const float4* __restrict__ col, float v, float n
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const float4 S = col[j], D = col[4 + j];
-        pux = pux + b[j] * D.x; puy = puy + b[j] * D.y; puz = puz + b[j] * D.z;
-        pvx = pvx + db[j] * S.x; pvy = pvy + db[j] * S.y; pvz = pvz + db[j] * S.z;
+        pux = mad(b[j], D.x, pux); puy = mad(b[j], D.y, puy); puz = mad(b[j], D.z, puz);
+        pvx = mad(db[j], S.x, pvx); pvy = mad(db[j], S.y, pvy); pvz = mad(db[j], S.z, pvz);
     }
-    float nx = pvy * puz - pvz * puy;
-    float ny = pvz * pux - pvx * puz;
-    float nz_ = pvx * puy - pvy * pux;
+    float nx = mad_diff(pvy, puz, pvz, puy);
+    float ny = mad_diff(pvz, pux, pvx, puz);
+    float nz_ = mad_diff(pvx, puy, pvy, pux);
     float len2 = dot3(nx, ny, nz_, nx, ny, nz_);
     n[0] = n[1] = n[2] = 0.0f;
     if (len2 > 0.0f) {
         float inv = 1.0f / sqrtf(len2);
         n[0] = nx * inv; n[1] = ny * inv; n[2] = nz_ * inv;
     }
+}
+
+// normal_from_columns with the column sums in LDS, component-major:
+// s_cols[(c * 8 + k) * G1 + nu] is component c (x, y, z) of col[k] of
+// column nu.  The lanes of a wave read one (c, k) at up to G1 <= 33
+// consecutive words, so no two columns of a read share an LDS bank (a
+// float4 per column and k, 128 bytes apart, put every column on the same
+// four banks).
+template <int G1>
+MP_DEV void normal_from_lds_columns(const float* __restrict__ s_cols, int nu, float v, float n[3])
+{
+    const float sv = 1.0f - v;
+    float b[4], db[4];
+    bern_v(v, sv, b);
+    dbern(v, sv, db);
+    const float* cx = s_cols + nu;
+    float pux = 0.f, puy = 0.f, puz = 0.f, pvx = 0.f, pvy = 0.f, pvz = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float Sx = cx[(0 * 8 + j) * G1], Sy = cx[(1 * 8 + j) * G1], Sz = cx[(2 * 8 + j) * G1];
+        const float Dx = cx[(0 * 8 + 4 + j) * G1], Dy = cx[(1 * 8 + 4 + j) * G1], Dz = cx[(2 * 8 + 4 + j) * G1];
+        pux = mad(b[j], Dx, pux); puy = mad(b[j], Dy, puy); puz = mad(b[j], Dz, puz);
+        pvx = mad(db[j], Sx, pvx); pvy = mad(db[j], Sy, pvy); pvz = mad(db[j], Sz, pvz);
+    }
+    float nx = mad_diff(pvy, puz, pvz, puy);
+    float ny = mad_diff(pvz, pux, pvx, puz);
+    float nz_ = mad_diff(pvx, puy, pvy, pux);
+    float len2 = dot3(nx, ny, nz_, nx, ny, nz_);
+    n[0] = n[1] = n[2] = 0.0f;
+    if (len2 > 0.0f) {
+        float inv = 1.0f / sqrtf(len2);
+        n[0] = nx * inv; n[1] = ny * inv; n[2] = nz_ * inv;
+    }
+}
+
+// displaced_point_cols with the column sums in LDS (normal_from_lds_columns).
+template <int G1>
+MP_DEV void displaced_point_lds_cols(const float4* __restrict__ cp, const float* __restrict__ s_cols, int nu,
+                                     float u, float v, const DevParams& prm, float amp, float out[4],
+                                     NoiseWindows& kept)
+{
+    eval_bezier(cp, u, v, out);
+    float h[3];
+    normal_from_lds_columns<G1>(s_cols, nu, v, h);
+    float d = amp * noise3_kept(prm, out[0], out[1], out[2], kept);
+    out[0] = mad(d, h[0], out[0]);
+    out[1] = mad(d, h[1], out[1]);
+    out[2] = mad(d, h[2], out[2]);
 }
 
 // Unit displacement direction at (u,v) (Bezier patches).
@@ -468,9 +548,9 @@ MP_DEV void displaced_point(const float4* __restrict__ cp, int type, float u, fl
     float h[3];
     bezier_unit_normal(cp, u, v, h);
     float d = amp * noise3(prm, out[0], out[1], out[2]);
-    out[0] = out[0] + d * h[0];
-    out[1] = out[1] + d * h[1];
-    out[2] = out[2] + d * h[2];
+    out[0] = mad(d, h[0], out[0]);
+    out[1] = mad(d, h[1], out[1]);
+    out[2] = mad(d, h[2], out[2]);
 }
 
 // Same with the normal's column sums precomputed (col: 8 float4 of the
@@ -485,9 +565,9 @@ MP_DEV void displaced_point_cols(const float4* __restrict__ cp, const float4* __
     normal_from_columns(col, v, h);
     float d = amp * (kept ? noise3_kept(prm, out[0], out[1], out[2], *kept)
                           : noise3(prm, out[0], out[1], out[2], full_wave));
-    out[0] = out[0] + d * h[0];
-    out[1] = out[1] + d * h[1];
-    out[2] = out[2] + d * h[2];
+    out[0] = mad(d, h[0], out[0]);
+    out[1] = mad(d, h[1], out[1]);
+    out[2] = mad(d, h[2], out[2]);
 }
 
 // ---- bound() (kernels/bound_n_split.h:43-100) with RES = 3 ----
@@ -507,8 +587,8 @@ MP_DEV void bound_sample(const float4 (&C)[NCP], int iu, int iv, float umin, flo
     mat4_mul(mv, p, ee);
     mat4_mul(P.proj, ee, q);
     const float rx = q[0] / q[3], ry = q[1] / q[3];
-    sx = P.screen[0] * rx + P.screen[2] * ry;
-    sy = P.screen[1] * rx + P.screen[3] * ry;
+    sx = mad(P.screen[2], ry, P.screen[0] * rx);
+    sy = mad(P.screen[3], ry, P.screen[1] * rx);
     e[0] = ee[0]; e[1] = ee[1]; e[2] = ee[2];
 }
 
@@ -534,10 +614,10 @@ MP_DEV int bound_decide(const float (&ex)[9], const float (&ey)[9], const float 
         if (f < P.near_ || n > P.far_) return MP_CULL;
         n = cl_max(P.near_, n);
         float hw = (float)P.width * 0.5f, hh = (float)P.height * 0.5f;
-        float sminx = bminx / ((bminx < 0) ? n : f) * P.f[0] + hw;
-        float sminy = bminy / ((bminy < 0) ? n : f) * P.f[1] + hh;
-        float smaxx = bmaxx / ((bmaxx > 0) ? n : f) * P.f[0] + hw;
-        float smaxy = bmaxy / ((bmaxy > 0) ? n : f) * P.f[1] + hh;
+        float sminx = mad(bminx / ((bminx < 0) ? n : f), P.f[0], hw);
+        float sminy = mad(bminy / ((bminy < 0) ? n : f), P.f[1], hh);
+        float smaxx = mad(bmaxx / ((bmaxx > 0) ? n : f), P.f[0], hw);
+        float smaxy = mad(bmaxy / ((bmaxy > 0) ? n : f), P.f[1], hh);
         const float rib = prm.cull_ribbon;
         if (sminx > (float)(P.width - 1) + rib || smaxx < -rib ||
             sminy > (float)(P.height - 1) + rib || smaxy < -rib)
@@ -554,10 +634,10 @@ MP_DEV int bound_decide(const float (&ex)[9], const float (&ey)[9], const float 
         for (int j = 0; j < 2; ++j) {
             float dx = sx[j * 3 + i] - sx[(j + 1) * 3 + i];
             float dy = sy[j * 3 + i] - sy[(j + 1) * 3 + i];
-            h = h + sqrtf(dx * dx + dy * dy);
+            h = h + sqrtf(mad(dy, dy, dx * dx));
             dx = sx[i * 3 + j] - sx[i * 3 + j + 1];
             dy = sy[i * 3 + j] - sy[i * 3 + j + 1];
-            v = v + sqrtf(dx * dx + dy * dy);
+            v = v + sqrtf(mad(dy, dy, dx * dx));
         }
         hlen = cl_max(h, hlen);
         vlen = cl_max(v, vlen);
@@ -661,7 +741,7 @@ MP_DEV bool inside_tri(int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t x
             int32_t den = wsub(wadd(wmul(Dx[j], Px[k]), wmul(Dy[j], Py[k])), O[j]);
             w[k] = (float)V[j] / (float)den;
         }
-        float d = (d0 * w[0] + d1 * w[1]) + d2 * w[2];
+        float d = mad(d2, w[2], mad(d1, w[1], d0 * w[0]));
         if (d < depth) depth = d;
     }
     return ok;
@@ -716,7 +796,7 @@ MP_DEV bool small_tri_test(const SmallTri& t, int32_t tx, int32_t ty, float d0, 
     }
     if (ok) {
         const float w0 = small_div((float)V[1], t), w1 = small_div((float)V[2], t), w2 = small_div((float)V[0], t);
-        const float d = (d0 * w0 + d1 * w1) + d2 * w2;
+        const float d = mad(d2, w2, mad(d1, w1, d0 * w0));
         if (d < depth) depth = d;
     }
     return ok;

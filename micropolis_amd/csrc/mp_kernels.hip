@@ -1424,9 +1424,9 @@ MP_DEV float quad_lambert(const float4& p0, const float4& p1, const float4& p2, 
     float dvy = ((p2.y - p0.y) + (p3.y - p1.y)) * 0.5f;
     float dvz = ((p2.z - p0.z) + (p3.z - p1.z)) * 0.5f;
     // n = cross(dv, du)
-    float nx = dvy * duz - dvz * duy;
-    float ny = dvz * dux - dvx * duz;
-    float nzz = dvx * duy - dvy * dux;
+    float nx = mad_diff(dvy, duz, dvz, duy);
+    float ny = mad_diff(dvz, dux, dvx, duz);
+    float nzz = mad_diff(dvx, duy, dvy, dux);
     normalize3(nx, ny, nzz);
     float lx = 4.0f, ly = 3.0f, lz = 8.0f;
     normalize3(lx, ly, lz);
@@ -1580,12 +1580,13 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
 // Dice one range (dice.cl:38-79) into LDS: eye-space position, sub-pixel
 // coordinate and depth of each of the (N+1)^2 grid points, displaced by the
 // value noise in MODE 1 (the displacement normal's column sums staged in
-// s_col first).  s_cp holds the range's control points.  Every lane of each
-// of the NT / 64 waves runs every pass (the lane-window noise exchanges
-// values across the wave); lanes past the last point repeat it and store
-// nothing.  nw: lattice windows the calling wave keeps across calls.
+// s_col first, 3 * 8 * (N+1) floats, component-major).  s_cp holds the
+// range's control points.  Every lane of each of the NT / 64 waves runs every
+// pass (the lane-window noise exchanges values across the wave); lanes past
+// the last point repeat it and store nothing.  nw: lattice windows the
+// calling wave keeps across calls.
 template <int N, int MODE, int NT, typename PT>
-MP_DEV void dice_grid(const float4* __restrict__ s_cp, float4* __restrict__ s_col, const RangeRec& rec,
+MP_DEV void dice_grid(const float4* __restrict__ s_cp, float* __restrict__ s_col, const RangeRec& rec,
                       const DevObject& ob, const DevView& vw, const DevParams& prm, PT* __restrict__ s_pos,
                       int2* __restrict__ s_pxl, float* __restrict__ s_dep, NoiseWindows& nw)
 {
@@ -1597,9 +1598,12 @@ MP_DEV void dice_grid(const float4* __restrict__ s_cp, float4* __restrict__ s_co
     if (disp_cols) {
         // displacement normal: column sums once per u column
         for (int t = tid; t < G1 * 8; t += NT) {
-            const int c = t >> 3;
+            const int c = t >> 3, k = t & 7;
             const float uc = rec.umin + (rec.umax - rec.umin) * ((float)c / (float)N);
-            s_col[t] = normal_column(s_cp, uc, t & 7);
+            const float4 v = normal_column(s_cp, uc, k);
+            s_col[(0 * 8 + k) * G1 + c] = v.x;
+            s_col[(1 * 8 + k) * G1 + c] = v.y;
+            s_col[(2 * 8 + k) * G1 + c] = v.z;
         }
         __syncthreads();
     }
@@ -1610,16 +1614,22 @@ MP_DEV void dice_grid(const float4* __restrict__ s_cp, float4* __restrict__ s_co
         const int g = min(gb + lane, G - 1);
         const int nu = g % G1, nv = g / G1;
         const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
-        const float uu = rec.umin + (rec.umax - rec.umin) * tu;
-        const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
+        // umin + (umax - umin) * tu (dice.cl:47): the width is a power of two
+        // and tu has at most 6 significant bits, so the product is exact and
+        // the fused form equals the written one
+        const float uu = mad(rec.umax - rec.umin, tu, rec.umin);
+        const float vv = mad(rec.vmax - rec.vmin, tv, rec.vmin);
         float P[4], e[4], q[4];
-        if (disp_cols) displaced_point_cols(s_cp, s_col + 8 * nu, uu, vv, prm, ob.disp_amplitude, P, true, &nw);
+        if (disp_cols) displaced_point_lds_cols<G1>(s_cp, s_col, nu, uu, vv, prm, ob.disp_amplitude, P, nw);
         else if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
         else eval_patch(s_cp, ob.type, uu, vv, P);
         mat4_mul(ob.mv, P, e);
-        mat4_mul(vw.proj, e, q);
-        const int32_t px = f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2));
-        const int32_t py = f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2));
+        perspective_mul(vw.proj, e, q);
+        // (x * W) / 2 + W / 2 (dice.cl:71-72): halving is exact, and a product
+        // too small for it to be is absorbed by the addend (>= 32), so the
+        // fused form equals the written one
+        const int32_t px = f2i_sat(mad((q[0] / q[3]) * (float)vw.vs_x, 0.5f, (float)(vw.vs_x / 2)));
+        const int32_t py = f2i_sat(mad((q[1] / q[3]) * (float)vw.vs_y, 0.5f, (float)(vw.vs_y / 2)));
         if (gb + lane < G) {
             set_pos(s_pos[g], e[0], e[1], e[2], e[3]);
             s_pxl[g] = make_int2(px, py);
@@ -1654,7 +1664,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ float s_box[NT / 64][6];
     __shared__ int s_geo[kCachedOctaves][6];
     __shared__ float s_lat[kCachedOctaves][kCacheEntries];
-    __shared__ float4 s_col[MODE == 1 ? G1 * 8 : 1];   // normal column sums per u column
+    __shared__ float s_col[MODE == 1 ? 3 * 8 * G1 : 1];   // normal column sums per u column
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -1801,13 +1811,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     } else {
                         nv = vnoise(p4.x * freq, p4.y * freq, p4.z * freq, prm.disp_seed + (uint32_t)k);
                     }
-                    acc = acc + amp * nv;
+                    acc = mad(amp, nv, acc);
                     norm = norm + amp;
                     amp = amp * 0.5f;
                     freq = freq * 2.0f;
                 }
                 const float d = ob.disp_amplitude * (acc / norm);
-                float P[4] = { p4.x + d * n4.x, p4.y + d * n4.y, p4.z + d * n4.z, p4.w };
+                float P[4] = { mad(d, n4.x, p4.x), mad(d, n4.y, p4.y), mad(d, n4.z, p4.z), p4.w };
                 float e[4], q[4];
                 mat4_mul(ob.mv, P, e);
                 mat4_mul(vw.proj, e, q);
@@ -1948,7 +1958,7 @@ __global__ __launch_bounds__(64) void k_read_grids(const RangeRec* __restrict__ 
     __shared__ float4 s_pos[G];
     __shared__ int2 s_pxl[G];
     __shared__ float s_dep[G];
-    __shared__ float4 s_col[MODE == 1 ? G1 * 8 : 1];
+    __shared__ float s_col[MODE == 1 ? 3 * 8 * G1 : 1];
     const int tid = threadIdx.x;
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
         const RangeRec rec = canon[first + k];
@@ -2044,7 +2054,7 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
             const float s = quad_store[(size_t)vw.range_base * Q + id];
             const DevObject& ob = objs[canon[rr].obj_depth & 0xffffffu];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) c[k] = (ac[k] * ob.color[k] + s * ob.color[k]) + 0.0f;
+            for (int k = 0; k < 4; ++k) c[k] = mad(s, ob.color[k], ac[k] * ob.color[k]) + 0.0f;
             bytes = display_byte(c[0], s_tab) | (display_byte(c[1], s_tab) << 8) |
                     (display_byte(c[2], s_tab) << 16) | (255u << 24);
         }
@@ -2318,7 +2328,7 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
                 // forward displaced eye position and pixel position: P + d * unit
                 // normal with the same operations as the forward dice
                 float Pd[4] = { P[0], P[1], P[2], P[3] };
-                if (prm.displacement) { Pd[0] = P[0] + d * h.x; Pd[1] = P[1] + d * h.y; Pd[2] = P[2] + d * h.z; }
+                if (prm.displacement) { Pd[0] = mad(d, h.x, P[0]); Pd[1] = mad(d, h.y, P[1]); Pd[2] = mad(d, h.z, P[2]); }
                 float e[4], q[4];
                 mat4_mul(ob.mv, Pd, e);
                 mat4_mul(vw.proj, e, q);

@@ -45,8 +45,16 @@
 #include <string.h>
 
 /* ------------------------------------------------------------------------ */
-/* OpenCL builtin semantics, spelled out (kernels use -cl-fast-relaxed-math;
- * this build fixes IEEE semantics).                                          */
+/* OpenCL builtin semantics, spelled out (kernels use -cl-fast-relaxed-math
+ * -cl-mad-enable, src/CL/Program.cpp:118; this build fixes IEEE semantics
+ * with one contraction rule for the kernel arithmetic: s + p*q is
+ * fmaf(p, q, s) and p*q + r*s (or -) is fmaf(r, s, p*q), the left product
+ * rounded -- the same places the HIP library fuses.  The CPU method's host
+ * code below is not contracted).                                             */
+
+/* s + p*q and p*q - r*s under the contraction rule */
+static inline float mad(float p, float q, float s) { return fmaf(p, q, s); }
+static inline float mad_diff(float p, float q, float r, float s) { return fmaf(-r, s, p * q); }
 
 /* OpenCL min(x,y): "Returns y if y < x, otherwise it returns x." */
 static inline float cl_minf(float x, float y) { return (y < x) ? y : x; }
@@ -74,14 +82,14 @@ static inline int32_t f2i_sat(float x)
 
 static inline float dot3(const float* a, const float* b)
 {
-    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+    return mad(a[2], b[2], mad(a[1], b[1], a[0] * b[0]));
 }
 
 static inline void cross3(const float* a, const float* b, float* o)
 {
-    o[0] = a[1] * b[2] - a[2] * b[1];
-    o[1] = a[2] * b[0] - a[0] * b[2];
-    o[2] = a[0] * b[1] - a[1] * b[0];
+    o[0] = mad_diff(a[1], b[2], a[2], b[1]);
+    o[1] = mad_diff(a[2], b[0], a[0], b[2]);
+    o[2] = mad_diff(a[0], b[1], a[1], b[0]);
 }
 
 /* normalize(v) = v * (1/sqrt(dot(v,v))) */
@@ -96,7 +104,7 @@ static inline void normalize3(const float* v, float* o)
 static inline void mat4_mul(const float* m, const float* v, float* o)
 {
     for (int i = 0; i < 4; ++i)
-        o[i] = ((m[0 + i] * v[0] + m[4 + i] * v[1]) + m[8 + i] * v[2]) + m[12 + i] * v[3];
+        o[i] = mad(m[12 + i], v[3], mad(m[8 + i], v[2], mad(m[4 + i], v[1], m[0 + i] * v[0])));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -122,10 +130,10 @@ static inline void bern_v(float t, float s, float* b)
 /* p += (a*b) * P, componentwise (utility.h:48 etc.) */
 static inline void acc4(float* p, float w, const float* P)
 {
-    p[0] = p[0] + w * P[0];
-    p[1] = p[1] + w * P[1];
-    p[2] = p[2] + w * P[2];
-    p[3] = p[3] + w * P[3];
+    p[0] = mad(w, P[0], p[0]);
+    p[1] = mad(w, P[1], p[1]);
+    p[2] = mad(w, P[2], p[2]);
+    p[3] = mad(w, P[3], p[3]);
 }
 
 static void eval_bezier(const float* cp, float u, float v, float* p)
@@ -156,10 +164,10 @@ static void eval_gregory(const float* P, float u, float v, float* p)
     float d2 = (2.0f - u) - v;
     float d3 = (1.0f + u) - v;
     for (int k = 0; k < 4; ++k) {
-        F0[k] = (u * P[4 * 12 + k] + v * P[4 * 16 + k]) / d0;
-        F1[k] = ((1.0f - u) * P[4 * 17 + k] + v * P[4 * 13 + k]) / d1;
-        F2[k] = ((1.0f - u) * P[4 * 14 + k] + (1.0f - v) * P[4 * 18 + k]) / d2;
-        F3[k] = (u * P[4 * 19 + k] + (1.0f - v) * P[4 * 15 + k]) / d3;
+        F0[k] = mad(v, P[4 * 16 + k], u * P[4 * 12 + k]) / d0;
+        F1[k] = mad(v, P[4 * 13 + k], (1.0f - u) * P[4 * 17 + k]) / d1;
+        F2[k] = mad(1.0f - v, P[4 * 18 + k], (1.0f - u) * P[4 * 14 + k]) / d2;
+        F3[k] = mad(1.0f - v, P[4 * 15 + k], u * P[4 * 19 + k]) / d3;
     }
     nan_to_zero4(F0); nan_to_zero4(F1); nan_to_zero4(F2); nan_to_zero4(F3);
 
@@ -241,7 +249,7 @@ static inline float lattice(int32_t x, int32_t y, int32_t z, uint32_t seed)
 }
 
 static inline float fade(float t) { return (t * t) * (3.0f - 2.0f * t); }
-static inline float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
+static inline float lerpf_(float a, float b, float t) { return mad(b - a, t, a); }
 
 static float vnoise(float x, float y, float z, uint32_t seed)
 {
@@ -262,7 +270,7 @@ float or_noise(const or_params* prm, float x, float y, float z)
 {
     float acc = 0.0f, amp = 1.0f, norm = 0.0f, freq = prm->disp_frequency;
     for (int k = 0; k < prm->disp_octaves; ++k) {
-        acc = acc + amp * vnoise(x * freq, y * freq, z * freq, prm->disp_seed + (uint32_t)k);
+        acc = mad(amp, vnoise(x * freq, y * freq, z * freq, prm->disp_seed + (uint32_t)k), acc);
         norm = norm + amp;
         amp = amp * 0.5f;
         freq = freq * 2.0f;
@@ -304,8 +312,8 @@ void or_displaced_point(const float* cps, int type, int pid, float u, float v,
             float sacc = 0.0f, dacc = 0.0f;
             for (int i = 0; i < 4; ++i) {
                 const float P = cp[4 * (4 * i + j) + k];
-                sacc = sacc + a[i] * P;
-                dacc = dacc + da[i] * P;
+                sacc = mad(a[i], P, sacc);
+                dacc = mad(da[i], P, dacc);
             }
             S[j][k] = sacc;
             D[j][k] = dacc;
@@ -313,8 +321,8 @@ void or_displaced_point(const float* cps, int type, int pid, float u, float v,
     float pu[3] = { 0, 0, 0 }, pv[3] = { 0, 0, 0 };
     for (int j = 0; j < 4; ++j)
         for (int k = 0; k < 3; ++k) {
-            pu[k] = pu[k] + b[j] * D[j][k];
-            pv[k] = pv[k] + db[j] * S[j][k];
+            pu[k] = mad(b[j], D[j][k], pu[k]);
+            pv[k] = mad(db[j], S[j][k], pv[k]);
         }
     float n[3];
     cross3(pv, pu, n);
@@ -325,9 +333,9 @@ void or_displaced_point(const float* cps, int type, int pid, float u, float v,
         nh[0] = n[0] * inv; nh[1] = n[1] * inv; nh[2] = n[2] * inv;
     }
     float d = prm->disp_amplitude * or_noise(prm, out[0], out[1], out[2]);
-    out[0] = out[0] + d * nh[0];
-    out[1] = out[1] + d * nh[1];
-    out[2] = out[2] + d * nh[2];
+    out[0] = mad(d, nh[0], out[0]);
+    out[1] = mad(d, nh[1], out[1]);
+    out[2] = mad(d, nh[2], out[2]);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -340,10 +348,10 @@ static int outside_frustum(const float* pmin, const float* pmax, const or_view* 
     if (f < P->near_ || n > P->far_) return 1;
     n = cl_maxf(P->near_, n);
     float hw = (float)P->width * 0.5f, hh = (float)P->height * 0.5f;
-    float sminx = pmin[0] / ((pmin[0] < 0) ? n : f) * P->f[0] + hw;
-    float sminy = pmin[1] / ((pmin[1] < 0) ? n : f) * P->f[1] + hh;
-    float smaxx = pmax[0] / ((pmax[0] > 0) ? n : f) * P->f[0] + hw;
-    float smaxy = pmax[1] / ((pmax[1] > 0) ? n : f) * P->f[1] + hh;
+    float sminx = mad(pmin[0] / ((pmin[0] < 0) ? n : f), P->f[0], hw);
+    float sminy = mad(pmin[1] / ((pmin[1] < 0) ? n : f), P->f[1], hh);
+    float smaxx = mad(pmax[0] / ((pmax[0] > 0) ? n : f), P->f[0], hw);
+    float smaxy = mad(pmax[1] / ((pmax[1] > 0) ? n : f), P->f[1], hh);
     return (sminx > (float)(P->width - 1) + ribbon || smaxx < -ribbon ||
             sminy > (float)(P->height - 1) + ribbon || smaxy < -ribbon);
 }
@@ -377,8 +385,8 @@ int or_bound(const float* cps, int type, int pid, int depth,
             mat4_mul(P->proj, e, q);
             float rx = q[0] / q[3], ry = q[1] / q[3];
             /* mul_cm2v2: m[0]*x + m[1]*y with m[0]=(s0,s1), m[1]=(s2,s3) */
-            ppos[iv][iu][0] = P->screen[0] * rx + P->screen[2] * ry;
-            ppos[iv][iu][1] = P->screen[1] * rx + P->screen[3] * ry;
+            ppos[iv][iu][0] = mad(P->screen[2], ry, P->screen[0] * rx);
+            ppos[iv][iu][1] = mad(P->screen[3], ry, P->screen[1] * rx);
         }
     }
 
@@ -392,10 +400,10 @@ int or_bound(const float* cps, int type, int pid, int depth,
         for (int j = 0; j < RES - 1; ++j) {
             float dx = ppos[i][j][0] - ppos[i][j + 1][0];
             float dy = ppos[i][j][1] - ppos[i][j + 1][1];
-            h = h + sqrtf(dx * dx + dy * dy);
+            h = h + sqrtf(mad(dy, dy, dx * dx));
             dx = ppos[j][i][0] - ppos[j + 1][i][0];
             dy = ppos[j][i][1] - ppos[j + 1][i][1];
-            v = v + sqrtf(dx * dx + dy * dy);
+            v = v + sqrtf(mad(dy, dy, dx * dx));
         }
         hlen = cl_maxf(h, hlen);
         vlen = cl_maxf(v, vlen);
@@ -1019,7 +1027,7 @@ static int inside_tri(const int32_t* Px, const int32_t* Py, const float* dv,
         int32_t den = wsub(wadd(wmul(Dx[j], Px[k]), wmul(Dy[j], Py[k])), O[j]);
         w[k] = (float)V[j] / (float)den;
     }
-    float d = (dv[0] * w[0] + dv[1] * w[1]) + dv[2] * w[2];
+    float d = mad(dv[2], w[2], mad(dv[1], w[1], dv[0] * w[0]));
     if (success && d < *depth) *depth = d;
     return success;
 }
@@ -1129,7 +1137,7 @@ void or_resolve(const uint64_t* keybuf, const float* quad_store, const float* ra
                 uint32_t r = id / (uint32_t)(N * N);
                 float s = quad_store[id];
                 const float* dc = range_color + 4 * (size_t)r;
-                for (int k = 0; k < 4; ++k) c[k] = (ac[k] * dc[k] + s * dc[k]) + 0.0f;
+                for (int k = 0; k < 4; ++k) c[k] = mad(s, dc[k], ac[k] * dc[k]) + 0.0f;
             }
             if (rgba8) {
                 rgba8[4 * o + 0] = or_to_srgb8(c[0]);
