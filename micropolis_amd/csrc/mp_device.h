@@ -50,6 +50,29 @@ MP_DEV void key_min(unsigned long long* p, unsigned long long v)
 
 namespace mp {
 
+// Wave votes over the active lanes: the ballot's lane mask against exec.
+// A ballot of one comparison is that comparison's lane mask; a compound
+// predicate is voted on per comparison (wave_all2 / wave_all3) so the masks
+// combine on the scalar unit instead of going through a vector select.
+MP_DEV uint64_t lane_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+MP_DEV bool wave_all(bool p) { return lane_mask(p) == __builtin_amdgcn_read_exec(); }
+MP_DEV bool wave_all2(bool p, bool q) { return (lane_mask(p) & lane_mask(q)) == __builtin_amdgcn_read_exec(); }
+MP_DEV bool wave_all3(bool p, bool q, bool r)
+{
+    return (lane_mask(p) & lane_mask(q) & lane_mask(r)) == __builtin_amdgcn_read_exec();
+}
+MP_DEV bool wave_any(bool p) { return lane_mask(p) != 0; }
+// Lane mask of v_cmp_class_f32 (the compiler turns finiteness tests into
+// class tests, whose ballots it would route through a vector select).
+constexpr int kClassFinite = 0x1F8;          // +-normal, +-subnormal, +-0
+constexpr int kClassFiniteNonZero = 0x198;   // +-normal, +-subnormal
+MP_DEV uint64_t class_mask(float x, int cls)
+{
+    uint64_t m;
+    asm("v_cmp_class_f32_e64 %0, %1, %2" : "=s"(m) : "v"(x), "s"(cls));
+    return m;
+}
+
 // OpenCL min(x,y): y if y < x else x.  OpenCL max(x,y): y if x < y else x.
 MP_DEV float cl_min(float x, float y) { return (y < x) ? y : x; }
 MP_DEV float cl_max(float x, float y) { return (x < y) ? y : x; }
@@ -58,13 +81,15 @@ MP_DEV int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint
 MP_DEV int32_t wsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
 MP_DEV int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
 
-// float -> int32: NaN -> 0, saturating, else truncation toward zero.
+// float -> int32: NaN -> 0, saturating, else truncation toward zero.  That
+// is v_cvt_i32_f32's own definition (out-of-range values and infinities
+// saturate, NaN converts to 0), so one instruction does it; C++'s (int)x
+// leaves those cases undefined and the compiler guards them with branches.
 MP_DEV int32_t f2i_sat(float x)
 {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return 2147483647;
-    if (x <= -2147483648.0f) return (int32_t)0x80000000u;
-    return (int32_t)x;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
 // s + p*q as one fused multiply-add (the contraction rule above)
@@ -95,9 +120,9 @@ MP_DEV void mat4_mul(const float* __restrict__ m, const float v[4], float o[4])
 // full product when any lane's point is not such a point.
 MP_DEV void perspective_mul(const float* __restrict__ m, const float e[4], float q[4])
 {
-    const bool plain = fabsf(e[0]) < INFINITY && fabsf(e[1]) < INFINITY && fabsf(e[2]) < INFINITY &&
-                       fabsf(e[3]) < INFINITY && e[2] != 0.0f && e[3] != 0.0f;
-    if (__all(plain)) {
+    const uint64_t plain = class_mask(e[0], kClassFinite) & class_mask(e[1], kClassFinite) &
+                           class_mask(e[2], kClassFiniteNonZero) & class_mask(e[3], kClassFiniteNonZero);
+    if (plain == __builtin_amdgcn_read_exec()) {
         q[0] = m[0] * e[0];
         q[1] = m[5] * e[1];
         q[2] = mad(m[14], e[3], m[10] * e[2]);
@@ -137,6 +162,17 @@ MP_DEV void acc4(float p[4], float w, const float4 P)
     p[3] = mad(w, P.w, p[3]);
 }
 
+// Bicubic Bezier from its Bernstein weights a (u) and b (v).
+MP_DEV void eval_bezier_w(const float4* __restrict__ cp, const float a[4], const float b[4], float p[4])
+{
+    p[0] = p[1] = p[2] = p[3] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            acc4(p, a[i] * b[j], cp[4 * i + j]);
+}
+
 // Bicubic Bezier (utility.h:41-72); cp = 16 control points (xyz, w=1).
 MP_DEV void eval_bezier(const float4* __restrict__ cp, float u, float v, float p[4])
 {
@@ -144,12 +180,7 @@ MP_DEV void eval_bezier(const float4* __restrict__ cp, float u, float v, float p
     float a[4], b[4];
     bern_u(u, su, a);
     bern_v(v, sv, b);
-    p[0] = p[1] = p[2] = p[3] = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            acc4(p, a[i] * b[j], cp[4 * i + j]);
+    eval_bezier_w(cp, a, b, p);
 }
 
 MP_DEV float nz(float x) { return (x != x) ? 0.0f : x; }
@@ -226,7 +257,9 @@ MP_DEV float lattice_wave(int32_t x, int32_t y, int32_t z, uint32_t seed)
     const float one_plus = __uint_as_float(0x3f800000u | (m & 0x7fffffu));
     return one_plus - ((m >> 23) ? 1.0f : 2.0f);
 }
-MP_DEV float fade(float t) { return (t * t) * (3.0f - 2.0f * t); }
+// (t * t) * (3 - 2t): 2t is exact, so 3 - 2t as one fma is the written
+// difference
+MP_DEV float fade(float t) { return (t * t) * __builtin_fmaf(-2.0f, t, 3.0f); }
 MP_DEV float lerpf_(float a, float b, float t) { return mad(b - a, t, a); }
 
 // Value of v held by lane byte_addr / 4 of the wave (ds_bpermute_b32).
@@ -253,7 +286,7 @@ MP_DEV bool window_corners(int32_t ix, int32_t iy, int32_t iz, uint32_t seed, fl
     const uint32_t by = (uint32_t)__builtin_amdgcn_readlane(iy, 32) - 1u;
     const uint32_t bz = (uint32_t)__builtin_amdgcn_readlane(iz, 32) - 1u;
     const uint32_t dx = (uint32_t)ix - bx, dy = (uint32_t)iy - by, dz = (uint32_t)iz - bz;
-    if (!__all(dx <= 2u && dy <= 2u && dz <= 2u)) return false;
+    if (!wave_all(dx <= 2u && dy <= 2u && dz <= 2u)) return false;
     const uint32_t k = threadIdx.x & 63u;
     const float hv = lattice((int32_t)(bx + (k & 3u)), (int32_t)(by + ((k >> 2) & 3u)),
                              (int32_t)(bz + (k >> 4)), seed);
@@ -278,7 +311,7 @@ MP_DEV float vnoise(float x, float y, float z, uint32_t seed, int scheme = kNois
     if (scheme == kNoiseLaneWindow && window_corners(ix, iy, iz, seed, c)) {
         c000 = c[0]; c100 = c[1]; c010 = c[2]; c110 = c[3];
         c001 = c[4]; c101 = c[5]; c011 = c[6]; c111 = c[7];
-    } else if (scheme == kNoiseWaveCell && __all(ix == jx && iy == jy && iz == jz)) {
+    } else if (scheme == kNoiseWaveCell && wave_all(ix == jx && iy == jy && iz == jz)) {
         // every active lane is in the same lattice cell: the eight corner
         // hashes are wave-uniform (scalar unit), the values are identical
         c000 = lattice_wave(jx, jy, jz, seed);         c100 = lattice_wave(jx + 1, jy, jz, seed);
@@ -298,47 +331,47 @@ MP_DEV float vnoise(float x, float y, float z, uint32_t seed, int scheme = kNois
 }
 
 // Lane windows kept by one wave across calls (the raster's dice passes and
-// the ranges after them, which mostly fall in the same lattice cells of the
-// low octaves): per octave the window's base corner and this lane's hashed
-// corner value.  A call whose cells all fit the kept window reads it with no
+// the ranges after them, which mostly fall in the same lattice cells): per
+// octave the window's base corner, kept in LDS (one uint4 per octave per
+// wave: x, y, z, and w = 1 once hashed), and this lane's hashed corner value
+// in a register.  A call whose cells all fit the kept window reads it with no
 // hashing; otherwise a new window around lane 32's cell is hashed and kept
 // (as window_corners does), and if even that does not fit, each lane hashes
 // its own eight corners.  The values are lattice()'s either way.  Whole
-// waves only.  Two octaves: keeping four costs the raster spilled registers.
-constexpr int kKeptOctaves = 2;
+// waves only; each wave owns its slots (LDS accesses of one wave are done in
+// order, so no barrier is needed between its calls).
+constexpr int kKeptOctaves = 4;
 struct NoiseWindows {
-    uint32_t bx[kKeptOctaves], by[kKeptOctaves], bz[kKeptOctaves];
+    uint4* base;   // LDS, kKeptOctaves slots of the calling wave
     float hv[kKeptOctaves];
-    bool valid[kKeptOctaves];
-    MP_DEV void reset()
+    MP_DEV void reset(uint4* slots)
     {
+        base = slots;
 #pragma unroll
         for (int k = 0; k < kKeptOctaves; ++k) {
-            bx[k] = by[k] = bz[k] = 0u;
             hv[k] = 0.0f;
-            valid[k] = false;
+            slots[k] = make_uint4(0u, 0u, 0u, 0u);
         }
     }
 };
 
-MP_DEV float vnoise_kept(float x, float y, float z, uint32_t seed, uint32_t& bx, uint32_t& by, uint32_t& bz,
-                         float& hv, bool& valid)
+MP_DEV float vnoise_kept(float x, float y, float z, uint32_t seed, uint4* slot, float& hv)
 {
     float fx = floorf(x), fy = floorf(y), fz = floorf(z);
     int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
     float wx = fade(x - fx), wy = fade(y - fy), wz = fade(z - fz);
-    uint32_t dx = (uint32_t)ix - bx, dy = (uint32_t)iy - by, dz = (uint32_t)iz - bz;
-    bool fits = valid && __all(dx <= 2u && dy <= 2u && dz <= 2u);
+    const uint4 B = *slot;
+    uint32_t dx = (uint32_t)ix - B.x, dy = (uint32_t)iy - B.y, dz = (uint32_t)iz - B.z;
+    bool fits = wave_all2(B.w != 0u, max(max(dx, dy), dz) <= 2u);
     if (!fits) {
         const uint32_t nx = (uint32_t)__builtin_amdgcn_readlane(ix, 32) - 1u;
         const uint32_t ny = (uint32_t)__builtin_amdgcn_readlane(iy, 32) - 1u;
         const uint32_t nz = (uint32_t)__builtin_amdgcn_readlane(iz, 32) - 1u;
         dx = (uint32_t)ix - nx; dy = (uint32_t)iy - ny; dz = (uint32_t)iz - nz;
-        if (__all(dx <= 2u && dy <= 2u && dz <= 2u)) {
+        if (wave_all(max(max(dx, dy), dz) <= 2u)) {
             const uint32_t k = threadIdx.x & 63u;
-            bx = nx; by = ny; bz = nz;
             hv = lattice((int32_t)(nx + (k & 3u)), (int32_t)(ny + ((k >> 2) & 3u)), (int32_t)(nz + (k >> 4)), seed);
-            valid = true;
+            *slot = make_uint4(nx, ny, nz, 1u);   // every lane stores the same value
             fits = true;
         }
     }
@@ -369,8 +402,8 @@ MP_DEV float noise3_kept(const DevParams& prm, float x, float y, float z, NoiseW
 #pragma unroll
     for (int k = 0; k < kKeptOctaves; ++k) {
         if (k < prm.disp_octaves) {
-            acc = mad(amp, vnoise_kept(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, w.bx[k],
-                                          w.by[k], w.bz[k], w.hv[k], w.valid[k]), acc);
+            acc = mad(amp, vnoise_kept(x * freq, y * freq, z * freq, prm.disp_seed + (uint32_t)k, w.base + k,
+                                       w.hv[k]), acc);
             norm = norm + amp;
             amp = amp * 0.5f;
             freq = freq * 2.0f;
@@ -404,10 +437,9 @@ MP_DEV float noise3(const DevParams& prm, float x, float y, float z, bool full_w
 // Column sums of the displacement normal at u, i ascending:
 // col[j] = sum_i a_i(u) P_ij, col[4 + j] = sum_i a_i'(u) P_ij (xyz; w = 0).
 // Every grid point of one u column shares them.
-MP_DEV float4 normal_column(const float4* __restrict__ cp, float u, int k)
+MP_DEV float4 normal_column(const float4* __restrict__ cp, float u, int k, float w[4])
 {
     const float su = 1.0f - u;
-    float w[4];
     if (k < 4) bern_u(u, su, w);
     else       dbern(u, su, w);
     const int j = k & 3;
@@ -418,6 +450,32 @@ MP_DEV float4 normal_column(const float4* __restrict__ cp, float u, int k)
         x = mad(w[i], P.x, x); y = mad(w[i], P.y, y); z = mad(w[i], P.z, z);
     }
     return make_float4(x, y, z, 0.0f);
+}
+
+// 1 / sqrtf(s), both correctly rounded.  Where every active lane's s lies
+// in [2^-80, 2^80] this is the IEEE expansions' own steps for that range:
+// v_sqrt_f32 with the neighbour test of the correctly rounded square root
+// (its scaling of operands below 2^-96 and its fix-up of zeros and infinity
+// are identities there), then the division's reciprocal refinement and
+// correction steps on the unscaled operands (v_div_scale and v_div_fixup
+// leave a numerator of 1 and a denominator in [2^-40, 2^40] unchanged).
+// Otherwise the wave takes '/' and sqrtf.
+MP_DEV float inv_sqrt_ieee(float s)
+{
+    if (wave_all2(s >= 0x1p-80f, s <= 0x1p80f)) {
+        float r = __builtin_amdgcn_sqrtf(s);
+        const float rd = __int_as_float(__float_as_int(r) - 1);
+        const float ru = __int_as_float(__float_as_int(r) + 1);
+        const float ed = __builtin_fmaf(-rd, r, s);
+        const float eu = __builtin_fmaf(-ru, r, s);
+        r = ed <= 0.0f ? rd : r;
+        r = eu > 0.0f ? ru : r;
+        const float y0 = __builtin_amdgcn_rcpf(r);
+        const float y = __builtin_fmaf(__builtin_fmaf(-r, y0, 1.0f), y0, y0);
+        const float q = __builtin_fmaf(__builtin_fmaf(-r, y, 1.0f), y, y);
+        return __builtin_fmaf(__builtin_fmaf(-r, q, 1.0f), y, q);
+    }
+    return 1.0f / sqrtf(s);
 }
 
 // Unit displacement direction from the column sums at v:
@@ -452,14 +510,12 @@ MP_DEV void normal_from_columns(const float4* __restrict__ col, float v, float n
 // column nu.  The lanes of a wave read one (c, k) at up to G1 <= 33
 // consecutive words, so no two columns of a read share an LDS bank (a
 // float4 per column and k, 128 bytes apart, put every column on the same
-// four banks).
+// four banks).  b, db: the Bernstein weights bern_v and dbern at the
+// point's v.
 template <int G1>
-MP_DEV void normal_from_lds_columns(const float* __restrict__ s_cols, int nu, float v, float n[3])
+MP_DEV void normal_from_lds_columns(const float* __restrict__ s_cols, int nu, const float b[4],
+                                    const float db[4], float n[3])
 {
-    const float sv = 1.0f - v;
-    float b[4], db[4];
-    bern_v(v, sv, b);
-    dbern(v, sv, db);
     const float* cx = s_cols + nu;
     float pux = 0.f, puy = 0.f, puz = 0.f, pvx = 0.f, pvy = 0.f, pvz = 0.f;
 #pragma unroll
@@ -475,20 +531,22 @@ MP_DEV void normal_from_lds_columns(const float* __restrict__ s_cols, int nu, fl
     float len2 = dot3(nx, ny, nz_, nx, ny, nz_);
     n[0] = n[1] = n[2] = 0.0f;
     if (len2 > 0.0f) {
-        float inv = 1.0f / sqrtf(len2);
+        float inv = inv_sqrt_ieee(len2);
         n[0] = nx * inv; n[1] = ny * inv; n[2] = nz_ * inv;
     }
 }
 
-// displaced_point_cols with the column sums in LDS (normal_from_lds_columns).
+// Displaced point of a Bezier patch with the normal's column sums in LDS
+// (normal_from_lds_columns) and the point's Bernstein weights given: a =
+// bern_u at u, b = bern_v and db = dbern at v.
 template <int G1>
 MP_DEV void displaced_point_lds_cols(const float4* __restrict__ cp, const float* __restrict__ s_cols, int nu,
-                                     float u, float v, const DevParams& prm, float amp, float out[4],
-                                     NoiseWindows& kept)
+                                     const float a[4], const float b[4], const float db[4],
+                                     const DevParams& prm, float amp, float out[4], NoiseWindows& kept)
 {
-    eval_bezier(cp, u, v, out);
+    eval_bezier_w(cp, a, b, out);
     float h[3];
-    normal_from_lds_columns<G1>(s_cols, nu, v, h);
+    normal_from_lds_columns<G1>(s_cols, nu, b, db, h);
     float d = amp * noise3_kept(prm, out[0], out[1], out[2], kept);
     out[0] = mad(d, h[0], out[0]);
     out[1] = mad(d, h[1], out[1]);
@@ -500,7 +558,10 @@ MP_DEV void bezier_unit_normal(const float4* __restrict__ cp, float u, float v, 
 {
     float4 col[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) col[k] = normal_column(cp, u, k);
+    for (int k = 0; k < 8; ++k) {
+        float w[4];
+        col[k] = normal_column(cp, u, k, w);
+    }
     normal_from_columns(col, v, n);
 }
 
@@ -548,23 +609,6 @@ MP_DEV void displaced_point(const float4* __restrict__ cp, int type, float u, fl
     float h[3];
     bezier_unit_normal(cp, u, v, h);
     float d = amp * noise3(prm, out[0], out[1], out[2]);
-    out[0] = mad(d, h[0], out[0]);
-    out[1] = mad(d, h[1], out[1]);
-    out[2] = mad(d, h[2], out[2]);
-}
-
-// Same with the normal's column sums precomputed (col: 8 float4 of the
-// point's u column).
-// kept: lattice windows the calling wave keeps across calls (whole waves).
-MP_DEV void displaced_point_cols(const float4* __restrict__ cp, const float4* __restrict__ col,
-                                 float u, float v, const DevParams& prm, float amp, float out[4],
-                                 bool full_wave = false, NoiseWindows* kept = nullptr)
-{
-    eval_bezier(cp, u, v, out);
-    float h[3];
-    normal_from_columns(col, v, h);
-    float d = amp * (kept ? noise3_kept(prm, out[0], out[1], out[2], *kept)
-                          : noise3(prm, out[0], out[1], out[2], full_wave));
     out[0] = mad(d, h[0], out[0]);
     out[1] = mad(d, h[1], out[1]);
     out[2] = mad(d, h[2], out[2]);

@@ -254,7 +254,7 @@ MP_DEV void with_object(const DevObject* __restrict__ objs, const DevView* __res
                         uint32_t obj, bool valid, F&& f)
 {
     const uint32_t o0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)obj);
-    if (__all(!valid || obj == o0)) {
+    if (wave_all(!valid || obj == o0)) {
         const DevObject& ob = objs[o0];
         f(ob, views[ob.view]);
     } else {
@@ -1412,6 +1412,12 @@ MP_DEV void normalize3(float& x, float& y, float& z)
     const float inv = 1.0f / sqrtf(dot3(x, y, z, x, y, z));
     x = x * inv; y = y * inv; z = z * inv;
 }
+// normalize3 of per-lane vectors (the same value, inv_sqrt_ieee's steps)
+MP_DEV void normalize3_wave(float& x, float& y, float& z)
+{
+    const float inv = inv_sqrt_ieee(dot3(x, y, z, x, y, z));
+    x = x * inv; y = y * inv; z = z * inv;
+}
 
 // Flat Lambert term of one micropolygon (reyes.cl:172-191); NaN when the
 // reference colour is NaN through the zero-weighted specular term.
@@ -1427,7 +1433,7 @@ MP_DEV float quad_lambert(const float4& p0, const float4& p1, const float4& p2, 
     float nx = mad_diff(dvy, duz, dvz, duy);
     float ny = mad_diff(dvz, dux, dvx, duz);
     float nzz = mad_diff(dvx, duy, dvy, dux);
-    normalize3(nx, ny, nzz);
+    normalize3_wave(nx, ny, nzz);
     float lx = 4.0f, ly = 3.0f, lz = 8.0f;
     normalize3(lx, ly, lz);
     const float s = cl_max(dot3(nx, ny, nzz, lx, ly, lz), 0.0f);
@@ -1480,8 +1486,8 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
                  p3 = eye4(s_pos[g0 + G1 + 1]);
     const int2 x0 = s_pxl[g0], x1 = s_pxl[g0 + 1], x2 = s_pxl[g0 + G1], x3 = s_pxl[g0 + G1 + 1];
     const float d0 = s_dep[g0], d1 = s_dep[g0 + 1], d2 = s_dep[g0 + G1], d3 = s_dep[g0 + G1 + 1];
-    const bool zero = (p0.z == 0.0f) || (p1.z == 0.0f) || (p2.z == 0.0f) || (p3.z == 0.0f);
-    const bool allnormal = !__any(zero);
+    const bool allnormal = (lane_mask(p0.z == 0.0f) | lane_mask(p1.z == 0.0f) | lane_mask(p2.z == 0.0f) |
+                            lane_mask(p3.z == 0.0f)) == 0;
     const int32_t vmaxx = (vw.width << sh) - 1, vmaxy = (vw.height << sh) - 1;
     bool ff = true;
     if (prm.backface_culling) {
@@ -1494,60 +1500,69 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
     const int32_t qminy = min(min(x0.y, x1.y), min(x2.y, x3.y));
     const int32_t qmaxx = max(max(x0.x, x1.x), max(x2.x, x3.x));
     const int32_t qmaxy = max(max(x0.y, x1.y), max(x2.y, x3.y));
+
+    quad_store[(size_t)r * Q + (u + v * N)] = quad_lambert(p0, p1, p2, p3);
+
+    // ---- sample (reyes.cl:270-398) ----
+    if (!allnormal) return;   // the reference empties the block's box
+    const uint32_t id = (r - vw.range_base) * (uint32_t)Q + (uint32_t)(u + v * N);
+    unsigned long long* keys = vw.keys;
+    const int gw = vw.grid_w;
+    // pixel-candidate bounds with the reference's initialisers
+    const int32_t mingx = min(vmaxx + 1, qminx), mingy = min(vmaxy + 1, qminy);
+    const int32_t maxgx = max(-1, qmaxx), maxgy = max(-1, qmaxy);
+    // Exact-integer regime: all coordinates within +-2^18 and edges
+    // shorter than 2^11 sub-pixels keep every edge product inside
+    // int32, so quads that are not front-facing and pixels outside a
+    // quad's bounding box can never pass the edge test.
+    const int32_t lim = 1 << 18;
+    const uint64_t small = lane_mask(qminx > -lim) & lane_mask(qmaxx < lim) & lane_mask(qminy > -lim) &
+                           lane_mask(qmaxy < lim) & lane_mask(qmaxx - qminx < (1 << 11)) &
+                           lane_mask(qmaxy - qminy < (1 << 11));
+    if (small == __builtin_amdgcn_read_exec()) {
+        if (ff) {
+            // The block's tile box (the wave's clamped quad boxes) only
+            // narrows a front-facing quad's pixels to the viewport and the
+            // tile after its last column / row: pixel X samples at its corner
+            // X << sh, inside the box only for X >= ceil(mingx / 2^sh) (the
+            // reference's loop starts at the floor, whose corner lies outside
+            // the box unless mingx is a whole pixel), from column 0, up to
+            // maxgx >> sh and the last column of the viewport's last tile.
+            // The box contains every front-facing quad's clamped box, so no
+            // wave reduction is needed here.
+            const int32_t rnd = (1 << sh) - 1;
+            const int32_t X0 = max((mingx + rnd) >> sh, 0), X1 = min(maxgx >> sh, (vw.width - 1) | 7);
+            const int32_t Y0 = max((mingy + rnd) >> sh, 0), Y1 = min(maxgy >> sh, (vw.height - 1) | 7);
+            SmallTri t1, t2;
+            small_tri_setup(t1, x0.x, x0.y, x1.x, x1.y, x3.x, x3.y);
+            small_tri_setup(t2, x0.x, x0.y, x3.x, x3.y, x2.x, x2.y);
+            for (int32_t Y = Y0; Y <= Y1; ++Y) {
+                for (int32_t X = X0; X <= X1; ++X) {
+                    const int32_t tx = X << sh, ty = Y << sh;
+                    float dd = 1.0f;
+                    const bool in1 = small_tri_test(t1, tx, ty, d0, d1, d3, dd);
+                    const bool in2 = small_tri_test(t2, tx, ty, d0, d3, d2, dd);
+                    if (in1 || in2) {
+                        const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
+                        key_min(&keys[fb_index(X, Y, gw)], key);
+                        ++natom;
+                    }
+                }
+            }
+        }
+        return;
+    }
+    // the block's tile box: front-facing quads' boxes, clamped to the viewport
     int32_t xmin = wave_min_i(ff ? min(vmaxx, qminx) : vmaxx);
     int32_t ymin = wave_min_i(ff ? min(vmaxy, qminy) : vmaxy);
     int32_t xmax = wave_max_i(ff ? max(0, qmaxx) : 0);
     int32_t ymax = wave_max_i(ff ? max(0, qmaxy) : 0);
     xmin = max(0, xmin); ymin = max(0, ymin);
     xmax = min(vmaxx, xmax); ymax = min(vmaxy, ymax);
-    if (!allnormal) { xmin = 1; ymin = 1; xmax = -1; ymax = -1; }
-
-    quad_store[(size_t)r * Q + (u + v * N)] = quad_lambert(p0, p1, p2, p3);
-
-    // ---- sample (reyes.cl:270-398) ----
     const int32_t mintx = xmin >> (sh + 3), minty = ymin >> (sh + 3);
     const int32_t maxtx = xmax >> (sh + 3), maxty = ymax >> (sh + 3);
     if (mintx <= maxtx && minty <= maxty) {
-        const uint32_t id = (r - vw.range_base) * (uint32_t)Q + (uint32_t)(u + v * N);
-        unsigned long long* keys = vw.keys;
-        const int gw = vw.grid_w;
-        // pixel-candidate bounds with the reference's initialisers
-        const int32_t mingx = min(vmaxx + 1, qminx), mingy = min(vmaxy + 1, qminy);
-        const int32_t maxgx = max(-1, qmaxx), maxgy = max(-1, qmaxy);
-        // Exact-integer regime: all coordinates within +-2^18 and edges
-        // shorter than 2^11 sub-pixels keep every edge product inside
-        // int32, so quads that are not front-facing and pixels outside a
-        // quad's bounding box can never pass the edge test.
-        const int32_t lim = 1 << 18;
-        const bool small = (qminx > -lim) && (qmaxx < lim) && (qminy > -lim) && (qmaxy < lim) &&
-                           (qmaxx - qminx < (1 << 11)) && (qmaxy - qminy < (1 << 11));
-        if (__all(small)) {
-            if (ff) {
-                // pixel X samples at its corner X << sh, inside the box only
-                // for X >= ceil(mingx / 2^sh); the reference's loop starts at
-                // the floor, whose corner lies outside the box (never passes)
-                // unless mingx is a whole pixel
-                const int32_t rnd = (1 << sh) - 1;
-                const int32_t X0 = max((mingx + rnd) >> sh, mintx * 8), X1 = min(maxgx >> sh, maxtx * 8 + 7);
-                const int32_t Y0 = max((mingy + rnd) >> sh, minty * 8), Y1 = min(maxgy >> sh, maxty * 8 + 7);
-                SmallTri t1, t2;
-                small_tri_setup(t1, x0.x, x0.y, x1.x, x1.y, x3.x, x3.y);
-                small_tri_setup(t2, x0.x, x0.y, x3.x, x3.y, x2.x, x2.y);
-                for (int32_t Y = Y0; Y <= Y1; ++Y) {
-                    for (int32_t X = X0; X <= X1; ++X) {
-                        const int32_t tx = X << sh, ty = Y << sh;
-                        float dd = 1.0f;
-                        const bool in1 = small_tri_test(t1, tx, ty, d0, d1, d3, dd);
-                        const bool in2 = small_tri_test(t2, tx, ty, d0, d3, d2, dd);
-                        if (in1 || in2) {
-                            const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
-                            key_min(&keys[fb_index(X, Y, gw)], key);
-                            ++natom;
-                        }
-                    }
-                }
-            }
-        } else {
+        {
             // literal per-tile loops of the reference (wide coordinates)
             if (lane == 0) ++slow;
             const int32_t maxl = (8 << sh) - 1;
@@ -1579,31 +1594,53 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
 
 // Dice one range (dice.cl:38-79) into LDS: eye-space position, sub-pixel
 // coordinate and depth of each of the (N+1)^2 grid points, displaced by the
-// value noise in MODE 1 (the displacement normal's column sums staged in
-// s_col first, 3 * 8 * (N+1) floats, component-major).  s_cp holds the
+// value noise in MODE 1.  For a displaced Bezier patch the per-range tables
+// come first: the displacement normal's column sums (s_col, 3 * 8 * (N+1)
+// floats, component-major) and the Bernstein weights of every u column and
+// v row (s_wt, 3 * (N+1) float4: bern_u of column c at s_wt[c], bern_v and
+// dbern of row r at s_wt[G1 + 2r] and s_wt[G1 + 2r + 1]), the values the
+// point evaluation would compute from the same u and v.  s_cp holds the
 // range's control points.  Every lane of each of the NT / 64 waves runs every
 // pass (the lane-window noise exchanges values across the wave); lanes past
 // the last point repeat it and store nothing.  nw: lattice windows the
 // calling wave keeps across calls.
 template <int N, int MODE, int NT, typename PT>
-MP_DEV void dice_grid(const float4* __restrict__ s_cp, float* __restrict__ s_col, const RangeRec& rec,
-                      const DevObject& ob, const DevView& vw, const DevParams& prm, PT* __restrict__ s_pos,
-                      int2* __restrict__ s_pxl, float* __restrict__ s_dep, NoiseWindows& nw)
+MP_DEV void dice_grid(const float4* __restrict__ s_cp, float* __restrict__ s_col, float4* __restrict__ s_wt,
+                      const RangeRec& rec, const DevObject& ob, const DevView& vw, const DevParams& prm,
+                      PT* __restrict__ s_pos, int2* __restrict__ s_pxl, float* __restrict__ s_dep,
+                      NoiseWindows& nw)
 {
     constexpr int G1 = N + 1;
     constexpr int G = G1 * G1;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const bool disp_cols = MODE == 1 && ob.type == MP_TYPE_BEZIER;
+    // umin + (umax - umin) * t (dice.cl:47): the width is a power of two
+    // and t = i / N has at most 6 significant bits, so the product is exact
+    // and the fused form equals the written one
+    const float du = rec.umax - rec.umin, dv = rec.vmax - rec.vmin;
     if (disp_cols) {
-        // displacement normal: column sums once per u column
-        for (int t = tid; t < G1 * 8; t += NT) {
-            const int c = t >> 3, k = t & 7;
-            const float uc = rec.umin + (rec.umax - rec.umin) * ((float)c / (float)N);
-            const float4 v = normal_column(s_cp, uc, k);
-            s_col[(0 * 8 + k) * G1 + c] = v.x;
-            s_col[(1 * 8 + k) * G1 + c] = v.y;
-            s_col[(2 * 8 + k) * G1 + c] = v.z;
+        for (int t = tid; t < G1 * 9; t += NT) {
+            if (t < G1 * 8) {
+                // displacement normal: column sums once per u column
+                const int c = t >> 3, k = t & 7;
+                const float uc = mad(du, (float)c / (float)N, rec.umin);
+                float w[4];
+                const float4 v = normal_column(s_cp, uc, k, w);
+                s_col[(0 * 8 + k) * G1 + c] = v.x;
+                s_col[(1 * 8 + k) * G1 + c] = v.y;
+                s_col[(2 * 8 + k) * G1 + c] = v.z;
+                if (k == 0) s_wt[c] = make_float4(w[0], w[1], w[2], w[3]);
+            } else {
+                const int r = t - G1 * 8;
+                const float vr = mad(dv, (float)r / (float)N, rec.vmin);
+                const float sv = 1.0f - vr;
+                float b[4], db[4];
+                bern_v(vr, sv, b);
+                dbern(vr, sv, db);
+                s_wt[G1 + 2 * r] = make_float4(b[0], b[1], b[2], b[3]);
+                s_wt[G1 + 2 * r + 1] = make_float4(db[0], db[1], db[2], db[3]);
+            }
         }
         __syncthreads();
     }
@@ -1613,16 +1650,18 @@ MP_DEV void dice_grid(const float4* __restrict__ s_cp, float* __restrict__ s_col
         asm volatile("" ::: "memory");
         const int g = min(gb + lane, G - 1);
         const int nu = g % G1, nv = g / G1;
-        const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
-        // umin + (umax - umin) * tu (dice.cl:47): the width is a power of two
-        // and tu has at most 6 significant bits, so the product is exact and
-        // the fused form equals the written one
-        const float uu = mad(rec.umax - rec.umin, tu, rec.umin);
-        const float vv = mad(rec.vmax - rec.vmin, tv, rec.vmin);
         float P[4], e[4], q[4];
-        if (disp_cols) displaced_point_lds_cols<G1>(s_cp, s_col, nu, uu, vv, prm, ob.disp_amplitude, P, nw);
-        else if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
-        else eval_patch(s_cp, ob.type, uu, vv, P);
+        if (disp_cols) {
+            const float4 A = s_wt[nu], Bv = s_wt[G1 + 2 * nv], Dv = s_wt[G1 + 2 * nv + 1];
+            const float a[4] = { A.x, A.y, A.z, A.w }, b[4] = { Bv.x, Bv.y, Bv.z, Bv.w };
+            const float db[4] = { Dv.x, Dv.y, Dv.z, Dv.w };
+            displaced_point_lds_cols<G1>(s_cp, s_col, nu, a, b, db, prm, ob.disp_amplitude, P, nw);
+        } else {
+            const float uu = mad(du, (float)nu / (float)N, rec.umin);
+            const float vv = mad(dv, (float)nv / (float)N, rec.vmin);
+            if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
+            else eval_patch(s_cp, ob.type, uu, vv, P);
+        }
         mat4_mul(ob.mv, P, e);
         perspective_mul(vw.proj, e, q);
         // (x * W) / 2 + W / 2 (dice.cl:71-72): halving is exact, and a product
@@ -1665,6 +1704,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ int s_geo[kCachedOctaves][6];
     __shared__ float s_lat[kCachedOctaves][kCacheEntries];
     __shared__ float s_col[MODE == 1 ? 3 * 8 * G1 : 1];   // normal column sums per u column
+    __shared__ float4 s_wt[MODE == 1 ? 3 * G1 : 1];       // Bernstein weights per u column / v row
+    __shared__ uint4 s_win[NT / 64][kKeptOctaves];         // kept lattice windows per wave
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -1702,7 +1743,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     int buf = 0;
     float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     NoiseWindows nw;   // lattice windows of the low octaves, kept across passes and ranges
-    nw.reset();
+    if (MODE == 1) nw.reset(s_win[wave]);
     while (r < total) {
         // next range (the following chunk is claimed one range ahead) and its
         // control points: loaded now, stored to the other LDS buffer after
@@ -1731,7 +1772,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
         // ---- dice (dice.cl:38-79) ----
         if (MODE != 2 || ob.type != MP_TYPE_BEZIER) {
-            dice_grid<N, MODE == 2 ? 0 : MODE, NT>(s_cp, s_col, rec, ob, vw, prm, s_pos, s_pxl, s_dep, nw);
+            dice_grid<N, MODE == 2 ? 0 : MODE, NT>(s_cp, s_col, s_wt, rec, ob, vw, prm, s_pos, s_pxl, s_dep, nw);
         } else if constexpr (MODE == 2) {
             // displaced: surface point and normal, then the value noise with
             // each octave's lattice values hashed once per range into LDS
@@ -1959,6 +2000,8 @@ __global__ __launch_bounds__(64) void k_read_grids(const RangeRec* __restrict__ 
     __shared__ int2 s_pxl[G];
     __shared__ float s_dep[G];
     __shared__ float s_col[MODE == 1 ? 3 * 8 * G1 : 1];
+    __shared__ float4 s_wt[MODE == 1 ? 3 * G1 : 1];
+    __shared__ uint4 s_win[kKeptOctaves];
     const int tid = threadIdx.x;
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
         const RangeRec rec = canon[first + k];
@@ -1967,8 +2010,8 @@ __global__ __launch_bounds__(64) void k_read_grids(const RangeRec* __restrict__ 
         if (tid < ncp) s_cp[tid] = ldg4(ob.cps + (size_t)rec.pid * ncp + tid);
         __syncthreads();
         NoiseWindows nw;
-        nw.reset();
-        dice_grid<N, MODE, 64>(s_cp, s_col, rec, ob, views[ob.view], prm, s_pos, s_pxl, s_dep, nw);
+        if (MODE == 1) nw.reset(s_win);
+        dice_grid<N, MODE, 64>(s_cp, s_col, s_wt, rec, ob, views[ob.view], prm, s_pos, s_pxl, s_dep, nw);
         __syncthreads();
         for (int g = tid; g < G; g += 64) {
             pos[(size_t)k * G + g] = s_pos[g];
