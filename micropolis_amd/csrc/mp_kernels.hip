@@ -1501,7 +1501,21 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
     const int32_t qmaxx = max(max(x0.x, x1.x), max(x2.x, x3.x));
     const int32_t qmaxy = max(max(x0.y, x1.y), max(x2.y, x3.y));
 
-    quad_store[(size_t)r * Q + (u + v * N)] = quad_lambert(p0, p1, p2, p3);
+    // Exact-integer regime: all coordinates within +-2^18 and edges
+    // shorter than 2^11 sub-pixels keep every edge product inside
+    // int32, so quads that are not front-facing and pixels outside a
+    // quad's bounding box can never pass the edge test.
+    const int32_t lim = 1 << 18;
+    const uint64_t small = lane_mask(qminx > -lim) & lane_mask(qmaxx < lim) & lane_mask(qminy > -lim) &
+                           lane_mask(qmaxy < lim) & lane_mask(qmaxx - qminx < (1 << 11)) &
+                           lane_mask(qmaxy - qminy < (1 << 11));
+    const bool exact = small == __builtin_amdgcn_read_exec();
+
+    // The Lambert term is read back (resolve, adjoint) only for micropolygons
+    // that own a pixel: none in a block whose box the reference empties, and
+    // in the exact-integer regime none that is not front-facing.  The other
+    // entries of the store are left as they were.
+    if (allnormal && (ff || !exact)) quad_store[(size_t)r * Q + (u + v * N)] = quad_lambert(p0, p1, p2, p3);
 
     // ---- sample (reyes.cl:270-398) ----
     if (!allnormal) return;   // the reference empties the block's box
@@ -1511,15 +1525,7 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
     // pixel-candidate bounds with the reference's initialisers
     const int32_t mingx = min(vmaxx + 1, qminx), mingy = min(vmaxy + 1, qminy);
     const int32_t maxgx = max(-1, qmaxx), maxgy = max(-1, qmaxy);
-    // Exact-integer regime: all coordinates within +-2^18 and edges
-    // shorter than 2^11 sub-pixels keep every edge product inside
-    // int32, so quads that are not front-facing and pixels outside a
-    // quad's bounding box can never pass the edge test.
-    const int32_t lim = 1 << 18;
-    const uint64_t small = lane_mask(qminx > -lim) & lane_mask(qmaxx < lim) & lane_mask(qminy > -lim) &
-                           lane_mask(qmaxy < lim) & lane_mask(qmaxx - qminx < (1 << 11)) &
-                           lane_mask(qmaxy - qminy < (1 << 11));
-    if (small == __builtin_amdgcn_read_exec()) {
+    if (exact) {
         if (ff) {
             // The block's tile box (the wave's clamped quad boxes) only
             // narrows a front-facing quad's pixels to the viewport and the
@@ -2388,8 +2394,11 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
         // micropolygon: gather dL/dC from the pixels it is visible in
         const int u = tid % N, v = tid / N, g0 = u + v * G1;
         const uint32_t myid = (r - vw.range_base) * (uint32_t)Q + (uint32_t)tid;
+        // the forward Lambert term (the store holds it for every
+        // micropolygon that owns a pixel; other entries are stale and unused)
         const float s_fwd = quad_store[(size_t)r * Q + tid];
         float gs = 0.0f, ga0 = 0.0f, ga1 = 0.0f, ga2 = 0.0f;
+        bool owns = false;
         if (s_fwd == s_fwd) {
             const int2 x0 = s_pxl[g0], x1 = s_pxl[g0 + 1], x2 = s_pxl[g0 + G1], x3 = s_pxl[g0 + G1 + 1];
             const int32_t qminx = min(min(x0.x, x1.x), min(x2.x, x3.x)), qmaxx = max(max(x0.x, x1.x), max(x2.x, x3.x));
@@ -2409,13 +2418,14 @@ __global__ __launch_bounds__(N * N) void k_adjoint(const RangeRec* __restrict__ 
                     for (int X = X0; X <= X1; ++X) {
                         const size_t o = (size_t)Y * vw.width + X;
                         if (gptr(vw.ids)[o] != myid) continue;
+                        owns = true;
                         const float* dl = dl_dcolor + ((size_t)vw.pixel_offset + o) * 4;
                         gs += dl[0] * ob.color[0] + dl[1] * ob.color[1] + dl[2] * ob.color[2];
                         ga0 += dl[0]; ga1 += dl[1]; ga2 += dl[2];
                     }
             }
         }
-        const float acs = 0.015f + (s_fwd == s_fwd ? s_fwd : 0.0f);
+        const float acs = owns ? 0.015f + s_fwd : 0.0f;
         ga0 *= acs; ga1 *= acs; ga2 *= acs;
         if (gs != 0.0f) {
             const float4 a0 = s_e[g0], a1 = s_e[g0 + 1], a2 = s_e[g0 + G1], a3 = s_e[g0 + G1 + 1];
