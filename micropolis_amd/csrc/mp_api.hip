@@ -97,6 +97,23 @@ struct mp_context {
     DevView* d_views = nullptr; DevView* h_views = nullptr; int views_cap = 0;
     DevObject* d_objs = nullptr; DevObject* h_objs = nullptr; int objs_cap = 0;
     FrameCounters* d_fc = nullptr; FrameCounters* h_fc = nullptr;
+    // the last frame's counters, copied by its resolve (k_resolve or
+    // k_frame_end) before it resets d_fc for the next frame
+    FrameCounters* d_fc_out = nullptr;
+    // d_fc, the split's shared state and the root queues are ready for a
+    // device frame to start with its split: armed by every frame end, and
+    // the root queues written for init_sig's layout
+    bool armed = false;
+    struct QueueSig {
+        int groups; uint32_t roots; int64_t stride; int balanced; const void* queue; const void* qcnt; const void* shq;
+        bool operator==(const QueueSig& o) const
+        {
+            return groups == o.groups && roots == o.roots && stride == o.stride && balanced == o.balanced &&
+                   queue == o.queue && qcnt == o.qcnt && shq == o.shq;
+        }
+    } init_sig = { -1, 0, 0, 0, nullptr, nullptr, nullptr };
+    bool graph_inits = false;   // the captured frame writes the root queues itself
+    bool frame_inits = false;   // the last device frame launched wrote them
     uint32_t* d_totals = nullptr; uint32_t* h_totals = nullptr;
 
     // persistent split queues
@@ -409,6 +426,7 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipMalloc((void**)&ctx->d_fc, sizeof(FrameCounters)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_fc_out, sizeof(FrameCounters)) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_fc, sizeof(FrameCounters), 0) != hipSuccess ||
         hipMalloc((void**)&ctx->d_totals, 4 * sizeof(uint32_t)) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_totals, 4 * sizeof(uint32_t), 0) != hipSuccess ||
@@ -474,7 +492,7 @@ extern "C" void mp_destroy(mp_context* ctx)
     for (auto& vb : ctx->views) free_view(vb);
     hipFree(ctx->d_views); hipHostFree(ctx->h_views);
     hipFree(ctx->d_objs); hipHostFree(ctx->h_objs);
-    hipFree(ctx->d_fc); hipHostFree(ctx->h_fc);
+    hipFree(ctx->d_fc); hipFree(ctx->d_fc_out); hipHostFree(ctx->h_fc);
     hipFree(ctx->d_totals); hipHostFree(ctx->h_totals);
     hipFree(ctx->d_display);
     hipFree(ctx->d_queue); hipFree(ctx->d_queue_cnt); hipFree(ctx->d_shared_q); hipFree(ctx->d_shared_cnt);
@@ -919,8 +937,25 @@ static SplitQueues split_queues(mp_context* ctx)
     return q;
 }
 
-// kernels that follow the split: canonical order, raster, resolve; an event
-// is recorded after each stage when `ev` is given
+// The frame-end step of this context (FrameEnd): counters to d_fc_out, and
+// the next device frame's counters and split state reset.
+static FrameEnd frame_end(mp_context* ctx)
+{
+    FrameEnd fe;
+    fe.fc = ctx->d_fc;
+    fe.out = ctx->d_fc_out;
+    fe.share = ctx->d_share;
+    fe.left_cnt = ctx->d_level_cnt ? ctx->d_level_cnt + 63 : nullptr;
+    if (ctx->cfg.bound_n_split_method == MP_BNS_BALANCED && ctx->d_shared_cnt) {
+        fe.shared_cnt = ctx->d_shared_cnt;
+        fe.shared_total = (int32_t)ctx->total_roots;
+    }
+    return fe;
+}
+
+// kernels that follow the split: canonical order, raster, resolve (which
+// also ends the frame); an event is recorded after each stage when `ev` is
+// given
 static int launch_back_end(mp_context* ctx, hipEvent_t* ev = nullptr)
 {
     CanonBuffers cb = canon_buffers(ctx);
@@ -934,15 +969,39 @@ static int launch_back_end(mp_context* ctx, hipEvent_t* ev = nullptr)
     if (!ctx->cfg.dummy_render)
         HIPCHK(ctx, launch_resolve(ctx->d_views, ctx->n_views, ctx->max_pixels, ctx->d_canon, ctx->d_objs,
                                    ctx->d_quad, ctx->cfg.reyes_patch_size, ctx->cfg.write_rgba32,
-                                   ctx->d_display, ctx->stream));
+                                   ctx->d_display, ctx->stream, frame_end(ctx)));
+    else
+        HIPCHK(ctx, launch_frame_end(frame_end(ctx), ctx->stream));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[2], ctx->stream));
+    ctx->armed = true;
     return MP_OK;
 }
 
-// the device-driven frame (LOCAL / BALANCED): no host synchronisation
+// the device-driven frame (LOCAL / BALANCED): no host synchronisation.  The
+// root queues are written (and the counters reset) only when the previous
+// frame did not leave them ready: a frame ends by resetting what its split
+// consumes, and the roots do not change between frames.
+static mp_context::QueueSig queue_sig(const mp_context* ctx)
+{
+    const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
+    const int64_t stride = (balanced && ctx->split_engine != 1)
+                               ? 0 : (int64_t)((ctx->total_roots + ctx->split_groups - 1) / ctx->split_groups);
+    return { ctx->split_groups, ctx->total_roots, stride, balanced, ctx->d_queue, ctx->d_queue_cnt, ctx->d_shared_q };
+}
+
+// whether a device frame may start with its split
+static bool frame_ready(const mp_context* ctx)
+{
+    return ctx->armed && queue_sig(ctx) == ctx->init_sig;
+}
+
 static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
 {
     const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
+    const mp_context::QueueSig sig = queue_sig(ctx);
+    const bool need_init = !frame_ready(ctx);
+    ctx->frame_inits = need_init;   // read by a capture of this frame
+    ctx->armed = false;   // until this frame's end
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
     // the first split launch resets the frame counters
     if (!ctx->total_roots) HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
@@ -959,15 +1018,21 @@ static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
         q.left = ctx->d_level_a;
         q.left_cnt = ctx->d_level_cnt + 63;
         q.left_cap = ctx->level_cap;
-        HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
-                                       balanced, ctx->d_fc, ctx->d_share, ctx->stream));
+        if (need_init) {
+            HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
+                                           balanced, ctx->d_fc, ctx->d_share, ctx->stream));
+            ctx->init_sig = sig;
+        }
         HIPCHK(ctx, launch_split_persistent(ctx->d_objs, ctx->d_views, ctx->prm, q,
                                             canon_buffers(ctx), ctx->d_fc, balanced, ctx->stream));
         HIPCHK(ctx, launch_split_rebalanced(ctx->d_objs, ctx->d_views, ctx->prm, q, canon_buffers(ctx),
                                             ctx->d_fc, ctx->stream));
     } else if (ctx->total_roots) {
-        HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
-                                       balanced, ctx->d_fc, ctx->d_share, ctx->stream));
+        if (need_init) {
+            HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q,
+                                           balanced, ctx->d_fc, ctx->d_share, ctx->stream));
+            ctx->init_sig = sig;
+        }
         HIPCHK(ctx, launch_split_persistent(ctx->d_objs, ctx->d_views, ctx->prm, q,
                                             canon_buffers(ctx), ctx->d_fc, balanced, ctx->stream));
     }
@@ -1204,7 +1269,10 @@ static int run_cpu_frame(mp_context* ctx, uint32_t& draw_total, int64_t& max_pat
     if (render)
         HIPCHK(ctx, launch_resolve(ctx->d_views, ctx->n_views, ctx->max_pixels, ctx->d_canon, ctx->d_objs,
                                    ctx->d_quad, ctx->cfg.reyes_patch_size, ctx->cfg.write_rgba32,
-                                   ctx->d_display, ctx->stream));
+                                   ctx->d_display, ctx->stream, frame_end(ctx)));
+    else
+        HIPCHK(ctx, launch_frame_end(frame_end(ctx), ctx->stream));
+    ctx->armed = true;
     draw_total = shipped;
     ctx->pass_counts = fr.pass_counts;
     max_patches = fr.max_stack;
@@ -1247,7 +1315,7 @@ static int execute_frame(mp_context* ctx)
         if ((rc = ensure_queues(ctx))) return rc;
         for (;;) {
             if ((rc = launch_device_frame(ctx))) return rc;
-            HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc_out, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
             const uint32_t ovf = ctx->h_fc->overflow;
             if (ovf & 4u) return fail(ctx, MP_E_HIP, "split work sharing timed out");
@@ -1284,13 +1352,13 @@ static int execute_frame(mp_context* ctx)
     } else if (method == MP_BNS_CPU) {
         HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
         if ((rc = run_cpu_frame(ctx, draw_total, max_patches))) return rc;
-        HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc_out, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     } else {
         HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
         if ((rc = run_segment_split(ctx, draw_total, dropped, max_patches))) return rc;
         if ((rc = launch_back_end(ctx))) return rc;
-        HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc_out, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -1661,20 +1729,28 @@ extern "C" int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* 
             if (rc) return rc;
         }
     } else if (use_graph) {
+        // a frame captured without the root-queue writes relies on the state
+        // the previous frame left
+        if (ctx->graph && !ctx->graph_inits && !frame_ready(ctx)) invalidate_graph(ctx);
         if (!ctx->graph) {
             hipGraph_t g = nullptr;
+            const bool was_armed = ctx->armed;   // a capture runs nothing
             HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             rc = launch_device_frame(ctx);
             hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            ctx->armed = was_armed;
             if (rc) { if (g) hipGraphDestroy(g); return rc; }
             HIPCHK(ctx, e);
             e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+            ctx->graph_inits = ctx->frame_inits;
             hipGraphDestroy(g);
             HIPCHK(ctx, e);
         }
+        ctx->armed = false;   // until the launches below have been enqueued
         HIPCHK(ctx, hipGraphLaunch(ctx->graph, ctx->stream));  // warm-up
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         for (int i = 0; i < n; ++i) HIPCHK(ctx, hipGraphLaunch(ctx->graph, ctx->stream));
+        ctx->armed = true;    // every replay ends with its frame-end step
     } else {
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         for (int i = 0; i < n; ++i)
@@ -1684,7 +1760,7 @@ extern "C" int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* 
     HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.0f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    HIPCHK(ctx, hipMemcpy(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(ctx->h_fc, ctx->d_fc_out, sizeof(FrameCounters), hipMemcpyDeviceToHost));
     if (ctx->h_fc->overflow) return fail(ctx, MP_E_STATE, "replay overflowed the draw list");
     ctx->counts_dirty = false;
     if (ms_per_frame) *ms_per_frame = (double)ms / n;
@@ -1730,7 +1806,7 @@ static int apply_tuning(mp_context* ctx, const mp_tuning& t)
     if ((rc = clear_stale_counts(ctx))) return rc;
     ctx->counts_dirty = true;
     if ((rc = launch_device_frame(ctx))) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc_out, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->h_fc->overflow) return fail(ctx, MP_E_STATE, "frame overflowed under this tuning");
     ctx->counts_dirty = false;
@@ -1923,7 +1999,7 @@ extern "C" int mp_backward(mp_context* ctx, uint64_t frame_serial, const float* 
     HIPCHK(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(grad_color), (size_t)no * 4, 0u, ctx->stream));
     HIPCHK(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(grad_amplitude), (size_t)no, 0u, ctx->stream));
     HIPCHK(ctx, launch_adjoint(ctx->cfg.reyes_patch_size, ctx->d_canon, ctx->d_objs, ctx->d_views, ctx->n_views,
-                               ctx->max_pixels, ctx->prm, ctx->d_fc, ctx->d_quad, dl_dcolor, ctx->d_visible,
+                               ctx->max_pixels, ctx->prm, ctx->d_fc_out, ctx->d_quad, dl_dcolor, ctx->d_visible,
                                ctx->d_visible + ctx->visible_cap, ctx->d_next, grad_cps, grad_color,
                                grad_amplitude, ctx->adjoint_grid_size, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));

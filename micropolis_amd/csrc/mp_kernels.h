@@ -103,9 +103,25 @@ hipError_t launch_raster(int patch_size, const RangeRec* canon, const DevObject*
 hipError_t launch_set_view_base(DevView* views, int view, uint32_t base, FrameCounters* fc, uint32_t total,
                                 hipStream_t s);
 int raster_grid(int patch_size, int num_cus, const DevParams& prm);
+// End of a frame: the frame's counters are copied to `out` (what the host
+// and the later kernels read) and reset for the next frame together with
+// the split's shared state (work-sharing chunks, the list of ranges left by a
+// step budget, the BALANCED root count), so the next device frame starts
+// directly with its split.  fc == nullptr: nothing is done.
+struct FrameEnd {
+    FrameCounters* fc = nullptr;
+    FrameCounters* out = nullptr;
+    ShareState* share = nullptr;
+    uint32_t* left_cnt = nullptr;     // reset to 0
+    int32_t* shared_cnt = nullptr;    // reset to shared_total
+    int32_t shared_total = 0;
+};
 hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           const RangeRec* canon, const DevObject* objs, const float* quad_store,
-                          int patch_size, int write_rgba32, const float* display_tab, hipStream_t s);
+                          int patch_size, int write_rgba32, const float* display_tab, hipStream_t s,
+                          const FrameEnd& fe = FrameEnd());
+// the frame-end step alone (frames that resolve nothing)
+hipError_t launch_frame_end(const FrameEnd& fe, hipStream_t s);
 
 hipError_t launch_init_roots(int obj, uint32_t n, RangeRec* out, hipStream_t s);
 

@@ -54,6 +54,28 @@ __global__ void k_reset(FrameCounters* fc, ShareState* share)
     if (threadIdx.x == 0) reset_counters(fc, share);
 }
 
+// One thread: FrameEnd's copy and resets (the frame's kernels before it have
+// finished with the counters).
+MP_DEV void end_frame(const FrameEnd& fe)
+{
+    if (!fe.fc) return;
+    *fe.out = *fe.fc;
+    reset_counters(fe.fc, fe.share);
+    if (fe.left_cnt) *fe.left_cnt = 0u;
+    if (fe.shared_cnt) *fe.shared_cnt = fe.shared_total;
+}
+
+__global__ void k_frame_end(FrameEnd fe)
+{
+    if (threadIdx.x == 0) end_frame(fe);
+}
+
+hipError_t launch_frame_end(const FrameEnd& fe, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_frame_end, dim3(1), dim3(64), 0, s, fe);
+    return hipGetLastError();
+}
+
 __global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2073,9 +2095,11 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
                                                  const RangeRec* __restrict__ canon,
                                                  const DevObject* __restrict__ objs,
                                                  const float* __restrict__ quad_store, int Q,
-                                                 int write_rgba32, const float* __restrict__ display_tab)
+                                                 int write_rgba32, const float* __restrict__ display_tab,
+                                                 FrameEnd fe)
 {
     __shared__ float s_tab[256];
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) end_frame(fe);   // the resolve reads no counter
     if (threadIdx.x < 255) s_tab[threadIdx.x] = display_tab[threadIdx.x];
     __syncthreads();
     const DevView& vw = views[blockIdx.y];
@@ -2116,7 +2140,8 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
 
 hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           const RangeRec* canon, const DevObject* objs, const float* quad_store,
-                          int patch_size, int write_rgba32, const float* display_tab, hipStream_t s)
+                          int patch_size, int write_rgba32, const float* display_tab, hipStream_t s,
+                          const FrameEnd& fe)
 {
     // one key per thread: every pixel's gather chain (range, object,
     // Lambert term) is in flight at once
@@ -2124,7 +2149,7 @@ hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
     if (bx > 65535) bx = 65535;
     if (bx < 1) bx = 1;
     hipLaunchKernelGGL(k_resolve, dim3(bx, n_views), dim3(256), 0, s, views, canon, objs,
-                       quad_store, patch_size * patch_size, write_rgba32, display_tab);
+                       quad_store, patch_size * patch_size, write_rgba32, display_tab, fe);
     return hipGetLastError();
 }
 
