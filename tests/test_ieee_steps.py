@@ -1,0 +1,181 @@
+"""The correction steps the kernels run after the hardware's first
+approximations, checked in exact rational arithmetic on the CPU.
+
+The kernels compute three correctly rounded FP32 operations without the
+compiler's full IEEE expansions, in the operand ranges where those expansions'
+scaling and fix-up steps are identities (mp_device.h ``inv_sqrt_ieee`` and
+``small_div``):
+
+  * sqrt(s) for s in [2^-80, 2^80]: v_sqrt_f32's result r (within 1 ulp) and
+    the neighbour test of the correctly rounded square root -- rd = r - 1 ulp,
+    ru = r + 1 ulp, take rd when fma(-rd, r, s) <= 0, then ru when
+    fma(-ru, r, s) > 0 (Tuckerman's rounding test: the sign of s - x^2 at the
+    neighbours decides the rounding).
+  * 1 / r for r in [2^-40, 2^40]: v_rcp_f32's y0 (within 1 ulp), the Newton
+    step y = fma(fma(-r, y0, 1), y0, y0), then two corrections
+    q = fma(fma(-r, q, 1), y, q) starting from q = y (Markstein's theorem: a
+    reciprocal within half an ulp and a quotient within one ulp give the
+    correctly rounded quotient after one residual correction).
+  * a / den for integers 0 <= a <= den < 2^23 (the exact-integer sampling
+    regime): the same steps with the refined reciprocal of den shared by the
+    three barycentric weights.
+
+Each case takes the first approximations the hardware may return (the
+correctly rounded value and its two neighbours) and compares the result with
+the correctly rounded square root or quotient computed exactly.  The square
+root's test corrects any of the three.  The division steps correct the two
+neighbours too, except for divisors whose 24-bit significand is all ones
+(Markstein's exception: there the Newton step keeps a one-ulp error); these
+steps are the compiler's own expansion of '/' (v_div_scale, v_div_fmas and
+v_div_fixup aside), so the kernels agree with the device's '/' for every
+operand, and the GPU parity tests hold the device's '/' equal to the oracle's
+on every diced point of the reference frames.  The integer quotients of the
+barycentric weights never meet the exception: their denominators (twice a
+triangle's area with edges below 2^11) stay below 2^23.  A planted sign error
+in a correction fails.
+"""
+from fractions import Fraction
+
+import numpy as np
+
+RNG = np.random.default_rng(20260)
+
+
+def _f32(x):
+    """Correctly rounded float32 of a rational x (round half to even)."""
+    if x == 0:
+        return np.float32(0.0)
+    sign = -1 if x < 0 else 1
+    x = abs(Fraction(x))
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    # scale so that the 24-bit significand is an integer part
+    while Fraction(2) ** e > x:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= x:
+        e += 1
+    e = max(e, -126)   # subnormals share the smallest exponent
+    scaled = x / Fraction(2) ** (e - 23)
+    m = scaled.numerator // scaled.denominator
+    rem = scaled - m
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and m % 2 == 1):
+        m += 1
+    return np.float32(sign * float(Fraction(m) * Fraction(2) ** (e - 23)))
+
+
+def _q(x):
+    return Fraction(float(x))
+
+
+def _fma(a, b, c):
+    return _f32(_q(a) * _q(b) + _q(c))
+
+
+def _step(x, n):
+    """x moved by n ulps (float32 bit pattern)."""
+    b = np.array([x], np.float32).view(np.int32)[0]
+    return np.array([b + n], np.int32).view(np.float32)[0]
+
+
+def _sqrt_steps(s, r):
+    rd, ru = _step(r, -1), _step(r, 1)
+    ed = _fma(-rd, r, s)
+    eu = _fma(-ru, r, s)
+    r = rd if ed <= 0 else r
+    return ru if eu > 0 else r
+
+
+def _recip_steps(d, y0):
+    y = _fma(_fma(-d, y0, np.float32(1.0)), y0, y0)
+    q = _fma(_fma(-d, y, np.float32(1.0)), y, y)
+    return _fma(_fma(-d, q, np.float32(1.0)), y, q)
+
+
+def _div_steps(a, den, r0):
+    rden = _fma(_fma(-den, r0, np.float32(1.0)), r0, r0)
+    q = _f32(_q(a) * _q(rden))
+    q = _fma(_fma(-den, q, a), rden, q)
+    return _fma(_fma(-den, q, a), rden, q)
+
+
+def _exact_sqrt(s):
+    # float64 sqrt rounded to float32 is the correctly rounded FP32 square
+    # root (53 >= 2 * 24 + 2: double rounding is innocuous for sqrt)
+    return np.float32(np.sqrt(np.float64(s)))
+
+
+def _samples(lo_exp, hi_exp, n):
+    e = RNG.uniform(lo_exp, hi_exp, n)
+    return (2.0 ** e).astype(np.float32)
+
+
+def test_sqrt_neighbour_test_is_correctly_rounded():
+    ss = np.concatenate([_samples(-80, 80, 400), np.float32([2.0 ** -80, 2.0 ** 80, 1.0, 2.0, 0.25])])
+    for s in ss:
+        want = _exact_sqrt(s)
+        for k in (-1, 0, 1):
+            got = _sqrt_steps(s, _step(want, k))
+            assert got == want, (s, k, got, want)
+
+
+def test_sqrt_test_catches_a_swapped_comparison():
+    # planted mistake: choosing rd when the residual is positive must fail
+    def wrong(s, r):
+        rd, ru = _step(r, -1), _step(r, 1)
+        r2 = rd if _fma(-rd, r, s) > 0 else r
+        return ru if _fma(-ru, r, s) > 0 else r2
+    bad = 0
+    for s in _samples(-20, 20, 100):
+        want = _exact_sqrt(s)
+        bad += any(wrong(s, _step(want, k)) != want for k in (-1, 0, 1))
+    assert bad > 0
+
+
+def _all_ones(d):
+    return (int(np.array([d], np.float32).view(np.int32)[0]) & 0x7FFFFF) == 0x7FFFFF
+
+
+def test_reciprocal_corrections_are_correctly_rounded():
+    ds = np.concatenate([_samples(-40, 40, 400), np.float32([2.0 ** -40, 2.0 ** 40, 1.0, 3.0, 0.7])])
+    for d in ds:
+        want = _f32(Fraction(1) / _q(d))
+        for k in ((0,) if _all_ones(d) else (-1, 0, 1)):
+            got = _recip_steps(d, _step(want, k))
+            assert got == want, (d, k, got, want)
+
+
+def test_all_ones_significands_need_the_correctly_rounded_start():
+    for e in range(-40, 40, 7):
+        d = np.float32((2.0 - 2.0 ** -23) * 2.0 ** e)
+        want = _f32(Fraction(1) / _q(d))
+        assert _recip_steps(d, want) == want
+        assert _recip_steps(d, _step(want, 1)) != want   # the exception is real
+
+
+def test_a_planted_sign_error_in_the_correction_fails():
+    def wrong(d, y0):
+        # every residual with the wrong sign
+        y = _fma(_fma(d, y0, np.float32(-1.0)), y0, y0)
+        q = _fma(_fma(d, y, np.float32(-1.0)), y, y)
+        return _fma(_fma(d, q, np.float32(-1.0)), y, q)
+    bad = 0
+    for d in _samples(-10, 10, 200):
+        want = _f32(Fraction(1) / _q(d))
+        if not _all_ones(d):
+            assert _recip_steps(d, _step(want, 1)) == want
+            bad += wrong(d, _step(want, 1)) != want
+    assert bad > 0
+
+
+def test_integer_quotients_of_the_barycentric_weights():
+    # twice a triangle's area in the exact-integer regime: below 2^23
+    dens = np.concatenate([RNG.integers(1, 1 << 23, 300), [1, 2, 3, (1 << 23) - 1, (1 << 22) + 1]])
+    for den in dens:
+        den_f = np.float32(den)
+        a_vals = [0, 1, int(den), int(RNG.integers(0, den + 1)), int(den) // 2]
+        r_true = _f32(Fraction(1) / Fraction(int(den)))
+        for a in a_vals:
+            a_f = np.float32(a)
+            want = _f32(Fraction(a, int(den)))
+            for k in (-1, 0, 1):
+                got = _div_steps(a_f, den_f, _step(r_true, k))
+                assert got == want, (a, den, k, got, want)
