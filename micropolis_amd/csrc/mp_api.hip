@@ -106,12 +106,13 @@ struct mp_context {
     bool armed = false;
     struct QueueSig {
         int groups; uint32_t roots; int64_t stride; int balanced; const void* queue; const void* qcnt; const void* shq;
+        const void* lcnt;   // the level counts (their last entry counts the ranges a step budget leaves)
         bool operator==(const QueueSig& o) const
         {
             return groups == o.groups && roots == o.roots && stride == o.stride && balanced == o.balanced &&
-                   queue == o.queue && qcnt == o.qcnt && shq == o.shq;
+                   queue == o.queue && qcnt == o.qcnt && shq == o.shq && lcnt == o.lcnt;
         }
-    } init_sig = { -1, 0, 0, 0, nullptr, nullptr, nullptr };
+    } init_sig = { -1, 0, 0, 0, nullptr, nullptr, nullptr, nullptr };
     bool graph_inits = false;   // the captured frame writes the root queues itself
     bool frame_inits = false;   // the last device frame launched wrote them
     uint32_t* d_totals = nullptr; uint32_t* h_totals = nullptr;
@@ -986,7 +987,8 @@ static mp_context::QueueSig queue_sig(const mp_context* ctx)
     const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
     const int64_t stride = (balanced && ctx->split_engine != 1)
                                ? 0 : (int64_t)((ctx->total_roots + ctx->split_groups - 1) / ctx->split_groups);
-    return { ctx->split_groups, ctx->total_roots, stride, balanced, ctx->d_queue, ctx->d_queue_cnt, ctx->d_shared_q };
+    return { ctx->split_groups, ctx->total_roots, stride, balanced, ctx->d_queue, ctx->d_queue_cnt, ctx->d_shared_q,
+             ctx->d_level_cnt };
 }
 
 // whether a device frame may start with its split
