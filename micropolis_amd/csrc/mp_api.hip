@@ -175,7 +175,11 @@ struct mp_context {
     uint32_t* d_next = nullptr;
     int adjoint_grid_size = 0;
     // level-synchronous split engine
-    int split_engine = 0;          // 0 persistent LDS-stack engine, 1 level-synchronous
+    int split_engine = 0;          // 0 persistent LDS-stack engine, 1 level-synchronous,
+                                   // 2 forest (a group splits whole trees with all its lanes)
+    int forest_cap = 2048;         // forest split: LDS stack entries per group
+    int forest_per_cu = 0;         // forest split: groups per CU (0: occupancy)
+    int patch_types = 0;           // bit 0: the frame has Bezier objects, bit 1: Gregory ones
     RangeRec* d_level_a = nullptr; RangeRec* d_level_b = nullptr; uint32_t level_cap = 0;
     uint32_t* d_level_cnt = nullptr; uint32_t* h_level_cnt = nullptr;
     int level_launches = 0;        // levels launched before the persistent tail
@@ -468,6 +472,8 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     if (const char* e = getenv("MP_FUSED_ORDER")) ctx->fused_order = atoi(e) != 0;
     if (const char* e = getenv("MP_SPLIT_WAVES")) ctx->split_waves = atoi(e) == 3 ? 3 : 2;
     if (const char* e = getenv("MP_SPLIT_ENGINE")) ctx->split_engine = atoi(e);
+    if (const char* e = getenv("MP_FOREST_CAP")) ctx->forest_cap = std::max(kForestMinStack, atoi(e));
+    if (const char* e = getenv("MP_FOREST_PER_CU")) ctx->forest_per_cu = std::max(0, atoi(e));
     if (const char* e = getenv("MP_CPU_THREADS")) ctx->cpu_threads = atoi(e);
     if (const char* e = getenv("MP_LEVEL_LANES")) ctx->level_lanes = atoi(e) == 1 ? 1 : 9;
     if (const char* e = getenv("MP_LEVEL_PER_CU")) ctx->level_per_cu = atoi(e);
@@ -780,6 +786,7 @@ static int upload_tables(mp_context* ctx)
                      [&](int a, int b) { return ctx->objects[a].view < ctx->objects[b].view; });
     std::vector<uint32_t> first(nv, 0), end(nv, 0);
     uint32_t g = 0;
+    ctx->patch_types = 0;
     int oi = 0;
     for (int v = 0; v < nv; ++v) {
         first[v] = g;
@@ -793,6 +800,7 @@ static int upload_tables(mp_context* ctx)
             d.split_limit = ctx->cfg.bound_n_split_limit;
             d.disp_amplitude = o.amplitude;
             d.type = ps.type;
+            ctx->patch_types |= ps.type == MP_GREGORY ? 2 : 1;
             d.n_patches = ps.n_patches;
             d.view = v;
             d.gpid_base = g;
@@ -843,12 +851,23 @@ static bool rebalancing(const mp_context* ctx)
            ctx->cfg.bound_n_split_method != MP_BNS_BALANCED && !ctx->work_sharing;
 }
 
+// BALANCED frames of the persistent engine share one root queue; every other
+// device-driven split deals the roots to private queues
+static bool shared_roots(const mp_context* ctx)
+{
+    return ctx->cfg.bound_n_split_method == MP_BNS_BALANCED && ctx->split_engine == 0;
+}
+
 static int ensure_queues(mp_context* ctx)
 {
     int rc;
-    const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
+    const int balanced = shared_roots(ctx);
     int groups = ctx->cfg.local_bns_work_groups;
-    if (groups <= 0) {
+    if (groups <= 0 && ctx->split_engine == 2) {
+        const int per_cu = ctx->forest_per_cu > 0 ? ctx->forest_per_cu
+                                                  : split_forest_occupancy(split_forest_lds(ctx->forest_cap));
+        groups = ctx->num_cus * per_cu;
+    } else if (groups <= 0) {
         groups = ctx->num_cus * split_persistent_occupancy(split_persistent_lds(ctx->stack_cap), ctx->split_waves);
     }
     ctx->split_groups = groups;
@@ -868,7 +887,7 @@ static int ensure_queues(mp_context* ctx)
     }
     // private root queues (ranges left by level launches or a step budget
     // are read from the level list in place)
-    const size_t stride = balanced && ctx->split_engine != 1 ? 1 : (ctx->total_roots + groups - 1) / groups;
+    const size_t stride = balanced ? 1 : (ctx->total_roots + groups - 1) / groups;
     const size_t entries = std::max<size_t>(1, stride * groups);
     if (entries > ctx->queue_entries) {
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -912,14 +931,13 @@ static int ensure_queues(mp_context* ctx)
 static SplitQueues split_queues(mp_context* ctx)
 {
     SplitQueues q;
-    const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
     q.queue = ctx->d_queue;
     q.queue_cnt = ctx->d_queue_cnt;
-    q.stride = (balanced && ctx->split_engine != 1) ? 0 : (int64_t)((ctx->total_roots + ctx->split_groups - 1) / ctx->split_groups);
+    q.stride = shared_roots(ctx) ? 0 : (int64_t)((ctx->total_roots + ctx->split_groups - 1) / ctx->split_groups);
     q.shared_q = ctx->d_shared_q;
     q.shared_cnt = ctx->d_shared_cnt;
     q.n_groups = ctx->split_groups;
-    q.stack_cap = ctx->stack_cap;
+    q.stack_cap = ctx->split_engine == 2 ? ctx->forest_cap : ctx->stack_cap;
     q.chunks = ctx->d_chunks;
     q.chunk_count = ctx->d_chunk_count;
     q.chunk_ready = ctx->d_chunk_ready;
@@ -984,9 +1002,8 @@ static int launch_back_end(mp_context* ctx, hipEvent_t* ev = nullptr)
 // consumes, and the roots do not change between frames.
 static mp_context::QueueSig queue_sig(const mp_context* ctx)
 {
-    const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
-    const int64_t stride = (balanced && ctx->split_engine != 1)
-                               ? 0 : (int64_t)((ctx->total_roots + ctx->split_groups - 1) / ctx->split_groups);
+    const int balanced = shared_roots(ctx);
+    const int64_t stride = balanced ? 0 : (int64_t)((ctx->total_roots + ctx->split_groups - 1) / ctx->split_groups);
     return { ctx->split_groups, ctx->total_roots, stride, balanced, ctx->d_queue, ctx->d_queue_cnt, ctx->d_shared_q,
              ctx->d_level_cnt };
 }
@@ -999,7 +1016,7 @@ static bool frame_ready(const mp_context* ctx)
 
 static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
 {
-    const int balanced = ctx->cfg.bound_n_split_method == MP_BNS_BALANCED;
+    const int balanced = shared_roots(ctx);
     const mp_context::QueueSig sig = queue_sig(ctx);
     const bool need_init = !frame_ready(ctx);
     ctx->frame_inits = need_init;   // read by a capture of this frame
@@ -1015,6 +1032,14 @@ static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
                                         ctx->total_roots, ctx->d_level_a, ctx->d_level_b, ctx->level_cap,
                                         ctx->d_level_cnt, ctx->level_launches, q, canon_buffers(ctx), ctx->d_fc,
                                         ctx->level_grid_size, ctx->level_lanes, ctx->d_share, ctx->stream));
+    } else if (ctx->total_roots && ctx->split_engine == 2) {
+        if (need_init) {
+            HIPCHK(ctx, launch_init_queues(ctx->d_objs, (int)ctx->objects.size(), ctx->total_roots, q, 0,
+                                           ctx->d_fc, ctx->d_share, ctx->stream));
+            ctx->init_sig = sig;
+        }
+        HIPCHK(ctx, launch_split_forest(ctx->d_objs, ctx->d_views, ctx->prm, q, canon_buffers(ctx), ctx->d_fc,
+                                        ctx->patch_types, (int)ctx->objects.size(), ctx->n_views, ctx->stream));
     } else if (ctx->total_roots && rebalancing(ctx)) {
         q.step_budget = ctx->split_budget;
         q.left = ctx->d_level_a;
@@ -1314,6 +1339,10 @@ static int execute_frame(mp_context* ctx)
 
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (device_driven) {
+        // the forest split stages the frame's tables in LDS: bigger tables
+        // take the persistent engine
+        if (ctx->split_engine == 2 && !split_forest_fits((int)ctx->objects.size(), ctx->n_views))
+            ctx->split_engine = 0;
         if ((rc = ensure_queues(ctx))) return rc;
         for (;;) {
             if ((rc = launch_device_frame(ctx))) return rc;
@@ -1331,7 +1360,12 @@ static int execute_frame(mp_context* ctx)
                 if ((rc = ensure_queues(ctx))) return rc;
             }
             ++relaunches;
-            if (ovf & 2u) {
+            if ((ovf & 2u) && ctx->split_engine == 2) {
+                // a forest group's trees outgrew its LDS stack: the
+                // persistent engine (which spreads ranges over groups)
+                ctx->split_engine = 0;
+                if ((rc = ensure_queues(ctx))) return rc;
+            } else if (ovf & 2u) {
                 const int full = 64 * (ctx->cfg.max_split_depth + 2);
                 if (ctx->stack_cap >= full) return fail(ctx, MP_E_INVALID, "split stack overflow");
                 ctx->stack_cap = full;   // deeper LDS stack; fewer resident groups
@@ -1790,7 +1824,9 @@ extern "C" int mp_get_tuning(mp_context* ctx, mp_tuning* t)
 // it once so that the level engine knows the frame's depth.
 static int apply_tuning(mp_context* ctx, const mp_tuning& t)
 {
-    ctx->split_engine = t.split_engine == 1 ? 1 : 0;
+    ctx->split_engine = (t.split_engine == 1 || t.split_engine == 2) ? t.split_engine : 0;
+    if (ctx->split_engine == 2 && !split_forest_fits((int)ctx->objects.size(), ctx->n_views))
+        return fail(ctx, MP_E_INVALID, "the forest split takes frames of at most 64 objects and views");
     ctx->work_sharing = t.work_sharing != 0;
     ctx->raster_grid_size = ctx->num_cus * std::max(1, t.raster_per_cu);
     ctx->prm.raster_chunk = t.raster_chunk;
@@ -1819,7 +1855,7 @@ static int apply_tuning(mp_context* ctx, const mp_tuning& t)
 extern "C" int mp_set_tuning(mp_context* ctx, const mp_tuning* t)
 {
     if (!ctx || !t) return MP_E_INVALID;
-    if (t->split_engine < 0 || t->split_engine > 1 || t->raster_per_cu < 0 || t->raster_per_cu > 64 ||
+    if (t->split_engine < 0 || t->split_engine > 2 || t->raster_per_cu < 0 || t->raster_per_cu > 64 ||
         t->raster_chunk < -1 || t->raster_chunk > 4096 || t->split_levels < 0 || t->split_levels > 23 ||
         t->split_budget < 0 || t->split_budget > 4096 ||
         (t->split_waves != 0 && t->split_waves != 2 && t->split_waves != 3))
@@ -1886,6 +1922,15 @@ extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
                 cand.push_back(c);
             }
         }
+    }
+    {                                   // forest split
+        mp_tuning c = cur;
+        c.split_engine = 2;
+        c.work_sharing = 0;
+        c.split_levels = 0;
+        c.split_budget = 0;
+        c.split_waves = 2;
+        cand.push_back(c);
     }
     for (int lv : { 0, 3, 5 }) {         // level launches, then the persistent tail
         mp_tuning c = cur;

@@ -617,9 +617,11 @@ MP_DEV void displaced_point(const float4* __restrict__ cp, int type, float u, fl
 // ---- bound() (kernels/bound_n_split.h:43-100) with RES = 3 ----
 // One of bound()'s 3x3 samples: eye-space point and screen position
 // (bound_n_split.h:52-68).
-template <int NCP>
+// V: DevView or any table with its proj, screen, f, near_, far_, eps, width
+// and height fields.
+template <int NCP, typename V>
 MP_DEV void bound_sample(const float4 (&C)[NCP], int iu, int iv, float umin, float vmin, float umax,
-                         float vmax, const float* __restrict__ mv, const DevView& P, float e[3],
+                         float vmax, const float* __restrict__ mv, const V& P, float e[3],
                          float& sx, float& sy)
 {
     const float tu = (float)iu * 0.5f, tv = (float)iv * 0.5f;
@@ -637,12 +639,15 @@ MP_DEV void bound_sample(const float4 (&C)[NCP], int iu, int iv, float umin, flo
 }
 
 // bound()'s decision from its nine samples, sample k = iu * 3 + iv
-// (bound_n_split.h:10-31, 70-100): box in sample order, frustum cull,
-// eye split, polyline lengths.
-MP_DEV int bound_decide(const float (&ex)[9], const float (&ey)[9], const float (&ez)[9],
-                        const float (&sx)[9], const float (&sy)[9], float umin, float vmin,
-                        float umax, float vmax, const DevView& P, const DevParams& prm,
-                        float split_limit)
+// (bound_n_split.h:10-31, 70-100), in three parts that bound_decide runs in
+// order and the forest split runs on separate lanes: the box in sample order
+// with the frustum cull and the eye split (bound_box_test: MP_CULL, a split,
+// or -1 to go on), the polyline lengths of one row i (bound_row: h along
+// samples (0, i), (1, i), (2, i), v along (i, 0), (i, 1), (i, 2)), and the
+// decision from the three rows' lengths (bound_rows_decide).
+template <typename V>
+MP_DEV int bound_box_test(const float (&ex)[9], const float (&ey)[9], const float (&ez)[9], float umin,
+                          float vmin, float umax, float vmax, const V& P, const DevParams& prm)
 {
     float bminx = INFINITY, bminy = INFINITY, bminz = INFINITY;
     float bmaxx = -INFINITY, bmaxy = -INFINITY, bmaxz = -INFINITY;
@@ -669,25 +674,54 @@ MP_DEV int bound_decide(const float (&ex)[9], const float (&ey)[9], const float 
     }
     if (bminz < P.eps && bmaxz > P.near_)
         return ((umax - umin) < (vmax - vmin)) ? MP_VSPLIT : MP_HSPLIT;
-    // px[iv][iu] = sx[iu * 3 + iv]
-    float hlen = 0.0f, vlen = 0.0f;
+    return -1;
+}
+
+// a0..a2: screen positions of samples (0, i), (1, i), (2, i); b0..b2: of
+// samples (i, 0), (i, 1), (i, 2)
+MP_DEV void bound_row(float ax0, float ay0, float ax1, float ay1, float ax2, float ay2, float bx0, float by0,
+                      float bx1, float by1, float bx2, float by2, float& h, float& v)
+{
+    const float ax[3] = { ax0, ax1, ax2 }, ay[3] = { ay0, ay1, ay2 };
+    const float bx[3] = { bx0, bx1, bx2 }, by[3] = { by0, by1, by2 };
+    h = 0.0f;
+    v = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        float h = 0.0f, v = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            float dx = sx[j * 3 + i] - sx[(j + 1) * 3 + i];
-            float dy = sy[j * 3 + i] - sy[(j + 1) * 3 + i];
-            h = h + sqrtf(mad(dy, dy, dx * dx));
-            dx = sx[i * 3 + j] - sx[i * 3 + j + 1];
-            dy = sy[i * 3 + j] - sy[i * 3 + j + 1];
-            v = v + sqrtf(mad(dy, dy, dx * dx));
-        }
-        hlen = cl_max(h, hlen);
-        vlen = cl_max(v, vlen);
+    for (int j = 0; j < 2; ++j) {
+        float dx = ax[j] - ax[j + 1];
+        float dy = ay[j] - ay[j + 1];
+        h = h + sqrtf(mad(dy, dy, dx * dx));
+        dx = bx[j] - bx[j + 1];
+        dy = by[j] - by[j + 1];
+        v = v + sqrtf(mad(dy, dy, dx * dx));
     }
+}
+
+MP_DEV int bound_rows_decide(float h0, float v0, float h1, float v1, float h2, float v2, float split_limit)
+{
+    float hlen = 0.0f, vlen = 0.0f;
+    hlen = cl_max(h0, hlen); vlen = cl_max(v0, vlen);
+    hlen = cl_max(h1, hlen); vlen = cl_max(v1, vlen);
+    hlen = cl_max(h2, hlen); vlen = cl_max(v2, vlen);
     if (hlen <= split_limit && vlen <= split_limit) return MP_DRAW;
     return (hlen > vlen) ? MP_VSPLIT : MP_HSPLIT;
+}
+
+template <typename V>
+MP_DEV int bound_decide(const float (&ex)[9], const float (&ey)[9], const float (&ez)[9],
+                        const float (&sx)[9], const float (&sy)[9], float umin, float vmin,
+                        float umax, float vmax, const V& P, const DevParams& prm,
+                        float split_limit)
+{
+    const int r = bound_box_test(ex, ey, ez, umin, vmin, umax, vmax, P, prm);
+    if (r >= 0) return r;
+    // px[iv][iu] = sx[iu * 3 + iv]
+    float h[3], v[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        bound_row(sx[i], sy[i], sx[3 + i], sy[3 + i], sx[6 + i], sy[6 + i], sx[i * 3], sy[i * 3], sx[i * 3 + 1],
+                  sy[i * 3 + 1], sx[i * 3 + 2], sy[i * 3 + 2], h[i], v[i]);
+    return bound_rows_decide(h[0], v[0], h[1], v[1], h[2], v[2], split_limit);
 }
 
 // The control points are loaded into registers once per range (NCP = 16

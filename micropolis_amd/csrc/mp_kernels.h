@@ -150,6 +150,19 @@ hipError_t launch_split_rebalanced(const DevObject* objs, const DevView* views, 
                                    hipStream_t s);
 hipError_t launch_xyz_to_cps(const float* xyz, float4* cps, size_t n, hipStream_t s);
 int split_persistent_occupancy(size_t lds_bytes, int waves);   // waves: 2 or 3 (SplitQueues::waves)
+// Forest split (LOCAL / BALANCED): group g splits the whole trees of the roots
+// in its private queue (k_init_queues' non-shared layout) on an LDS stack of
+// q.stack_cap entries (at least kForestMinStack).
+// patch_types: bit 0 some object has Bezier patches, bit 1 some has Gregory
+// patches (the kernel build without the other evaluation is launched)
+hipError_t launch_split_forest(const DevObject* objs, const DevView* views, const DevParams& prm,
+                               const SplitQueues& q, const CanonBuffers& cb, FrameCounters* fc, int patch_types,
+                               int n_objs, int n_views, hipStream_t s);
+// whether a frame's object and view tables fit the forest split's LDS copy
+bool split_forest_fits(int n_objs, int n_views);
+int split_forest_occupancy(size_t lds_bytes);
+size_t split_forest_lds(int stack_cap);
+constexpr int kForestMinStack = 512;
 size_t split_persistent_lds(int stack_cap);   // LDS bytes of one persistent split group
 
 }  // namespace mp

@@ -827,6 +827,365 @@ hipError_t launch_split_persistent(const DevObject* objs, const DevView* views,
 }
 
 // ------------------------------------------------------------------------
+// LOCAL / BALANCED: forest split
+//
+// A work-group owns the whole split trees of its roots (root g of the frame
+// belongs to group g mod G, the private queue layout of k_init_queues) and
+// works on them with all NT lanes: a step pops up to kForestStep ranges from
+// the group's LDS stack (roots are taken from the private queue while the
+// stack holds fewer), spreads bound()'s nine samples of every popped range
+// over the lanes (on the narrowest steps the 21 samples of the range and of
+// its possible children, deciding two levels at once as the persistent
+// kernel's spec_sample steps do), decides each range on one lane from the
+// samples in LDS and pushes the children of split ranges back with one
+// work-group prefix count.  No range leaves its group, so a step touches
+// global memory only for control points and the buffered DRAW appends, and a
+// group runs about one step per level of its trees.  Every lane runs the same
+// sample code and the same decision code in every step (one copy of each in
+// the kernel: the code of the whole kernel stays inside the instruction
+// cache), with the object and view tables read per lane; PT selects the
+// patch types compiled in (0 Bezier, 1 Gregory, 2 both).
+constexpr int kForestThreads = 512;
+
+// spec_sample and spec_range with every field chosen by selects (a record
+// whose fields are stored by a runtime index goes through scratch memory)
+MP_DEV void spec_sample_sel(const RangeRec& p, int j, RangeRec& c, int& iu, int& iv)
+{
+    const float cu = (p.umin + p.umax) * 0.5f, cv = (p.vmin + p.vmax) * 0.5f;   // as split_children
+    c.pid = p.pid;
+    c.obj_depth = p.obj_depth + (j >= 9 ? (1u << 24) : 0u);
+    c.umin = (j >= 12 && j < 15) ? cu : p.umin;
+    c.umax = (j >= 9 && j < 12) ? cu : p.umax;
+    c.vmin = j >= 18 ? cv : p.vmin;
+    c.vmax = (j >= 15 && j < 18) ? cv : p.vmax;
+    const int r = j < 9 ? j / 3 : (j < 15 ? 1 : (j - 15) % 3);
+    iv = j < 9 ? j - 3 * r : (j < 15 ? (j - 9) % 3 : 1);
+    iu = r;
+}
+
+MP_DEV void spec_range_sel(const RangeRec& p, int d, RangeRec& c)
+{
+    const float cu = (p.umin + p.umax) * 0.5f, cv = (p.vmin + p.vmax) * 0.5f;
+    c.pid = p.pid;
+    c.obj_depth = p.obj_depth + (d != 0 ? (1u << 24) : 0u);
+    c.umin = d == 2 ? cu : p.umin;
+    c.umax = d == 1 ? cu : p.umax;
+    c.vmin = d == 4 ? cv : p.vmin;
+    c.vmax = d == 3 ? cv : p.vmax;
+}
+
+// The object and view fields the split reads, staged in LDS once per
+// work-group (frames of at most kForestTables objects and views).
+constexpr int kForestTables = 64;
+struct ForestObj {
+    float mv[16];
+    const float4* cps;
+    float split_limit;
+    int32_t type;
+    int32_t view;
+    uint32_t gpid_base;
+};
+struct ForestView {
+    float proj[16];
+    float screen[4];
+    float f[2];
+    float near_, far_, eps;
+    int32_t width, height;
+};
+// Control points of a group's roots staged in LDS: root slot k (the group's
+// k-th root, global patch index wid + k * G) at k * stride float4.
+constexpr int kForestCps = 2048;   // float4
+
+// bound() sample (iu, iv) of rec: eye-space x, y, z and screen x, y; the
+// range's control points from the group's LDS copy when its root has a slot
+template <int PT>
+MP_DEV void forest_sample(const ForestObj* s_obj, const ForestView* s_view, const float4* s_cps, int slots,
+                          int stride, const RangeRec& rec, int iu, int iv, float (&v)[5])
+{
+    const ForestObj& ob = s_obj[rec.obj_depth & 0xffffffu];
+    const ForestView& P = s_view[ob.view];
+    const uint32_t k = (ob.gpid_base + (uint32_t)rec.pid - (uint32_t)blockIdx.x) / gridDim.x;
+    const bool cached = k < (uint32_t)slots;
+    float e[3];
+    if (PT == 1 || (PT == 2 && ob.type == MP_TYPE_GREGORY)) {
+        const float4* cp = ob.cps + (size_t)rec.pid * 20;
+        float4 C[20];
+        if (cached) {
+#pragma unroll
+            for (int i = 0; i < 20; ++i) C[i] = s_cps[k * stride + i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 20; ++i) C[i] = ldg4(cp + i);
+        }
+        bound_sample<20>(C, iu, iv, rec.umin, rec.vmin, rec.umax, rec.vmax, ob.mv, P, e, v[3], v[4]);
+    } else {
+        const float4* cp = ob.cps + (size_t)rec.pid * 16;
+        float4 C[16];
+        if (cached) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) C[i] = s_cps[k * stride + i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) C[i] = ldg4(cp + i);
+        }
+        bound_sample<16>(C, iu, iv, rec.umin, rec.vmin, rec.umax, rec.vmax, ob.mv, P, e, v[3], v[4]);
+    }
+    v[0] = e[0]; v[1] = e[1]; v[2] = e[2];
+}
+
+// Flushes the group's buffered DRAW ranges (buf[0 .. cnt), cnt <= NT) to the
+// frame's draw list under one counter atomic and counts them per patch; all
+// NT threads call it.  Returns false (uniformly) when the list overflows.
+template <int NT>
+MP_DEV bool forest_flush(const DevObject* __restrict__ objs, const CanonBuffers& cb, FrameCounters* fc,
+                         const RangeRec* buf, int cnt, uint32_t* s_base)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) *s_base = atomicAdd(&fc->draw_count, (uint32_t)cnt);
+    __syncthreads();
+    const uint32_t base = *s_base;
+    const bool have = tid < cnt;
+    const RangeRec r = buf[min(tid, max(cnt - 1, 0))];
+    const uint32_t idx = base + (uint32_t)tid;
+    const bool put = have && idx < cb.cap;
+    if (put) cb.draw[idx] = r;
+    wave_count_add(cb.counts, put ? gpid_of(objs, r) : 0u, put);
+    __syncthreads();   // buffer and base read before they are reused
+    if (base + (uint32_t)cnt > cb.cap) {
+        if (tid == 0) atomicOr(&fc->overflow, 1u);
+        return false;
+    }
+    return true;
+}
+
+template <int NT, int PT>
+__global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict__ objs,
+                                                     const DevView* __restrict__ views, DevParams prm,
+                                                     SplitQueues q, CanonBuffers cb, FrameCounters* fc,
+                                                     int n_objs, int n_views)
+{
+    static_assert(NT % 64 == 0 && NT >= 256, "whole waves");
+    constexpr int NW = NT / 64;
+    constexpr int STEP = NT / 2;                   // ranges per step
+    constexpr int SPEC = NT / kSpecSamples;        // ranges of a two-level step
+    static_assert(5 * SPEC <= NT, "two-level candidates");
+    constexpr int DBUF = NT;                       // buffered DRAW ranges
+    extern __shared__ RangeRec s_stack[];          // q.stack_cap entries
+    __shared__ float s_xch[9 * STEP * 5];          // the step's samples, 5 floats each
+    __shared__ RangeRec s_dbuf[DBUF];              // DRAW ranges not yet appended
+    __shared__ int s_wcnt[NW][2];                  // per wave: split / draw lanes
+    __shared__ int s_pf[SPEC + 1];                 // two-level steps: the ranges' decisions
+    __shared__ uint32_t s_base;
+    __shared__ ForestObj s_obj[kForestTables];
+    __shared__ ForestView s_view[kForestTables];
+    __shared__ float4 s_cps[kForestCps];
+    static_assert(kSpecSamples * SPEC <= 9 * STEP, "sample buffer");
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wid = blockIdx.x;
+    const int cap = q.stack_cap;
+    const int dlim = prm.max_split_depth - 1;      // bound() culls ranges this deep
+    const RangeRec* myq = q.queue + (int64_t)wid * q.stride;
+    int qcnt = q.queue_cnt[wid];
+    int sh = 0, dfill = 0, maxsh = 0;
+    uint32_t nbounds = 0;
+    bool ok = true;
+
+    // tables and the roots' control points to LDS
+    for (int i = tid; i < n_objs; i += NT) {
+        const DevObject& o = objs[i];
+        ForestObj f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) f.mv[k] = o.mv[k];
+        f.cps = o.cps;
+        f.split_limit = o.split_limit;
+        f.type = o.type;
+        f.view = o.view;
+        f.gpid_base = o.gpid_base;
+        s_obj[i] = f;
+    }
+    for (int i = tid; i < n_views; i += NT) {
+        const DevView& v = views[i];
+        ForestView f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) f.proj[k] = v.proj[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) f.screen[k] = v.screen[k];
+        f.f[0] = v.f[0]; f.f[1] = v.f[1];
+        f.near_ = v.near_; f.far_ = v.far_; f.eps = v.eps;
+        f.width = v.width; f.height = v.height;
+        s_view[i] = f;
+    }
+    constexpr int CS = PT == 0 ? 16 : 20;   // float4 per root slot
+    const int slots = min(qcnt, kForestCps / CS);
+    for (int i = tid; i < slots * CS; i += NT) {
+        const int k = i / CS, c = i - CS * (i / CS);
+        // slot k: the root of global patch index wid + k * G (k_init_queues)
+        const RangeRec rt = myq[k];
+        const DevObject& o = objs[rt.obj_depth & 0xffffffu];
+        const int ncp = o.type == MP_TYPE_GREGORY ? 20 : 16;
+        if (c < ncp) s_cps[i] = ldg4(o.cps + (size_t)rt.pid * ncp + c);
+    }
+
+    for (;;) {
+        // roots from the private queue while the step would not be full
+        if (sh < STEP && qcnt > 0) {
+            const int take = min(STEP - sh, qcnt);
+            qcnt -= take;
+            if (tid < take) s_stack[sh + tid] = myq[qcnt + tid];
+            sh += take;
+        }
+        __syncthreads();   // roots and the last step's pushes visible
+        if (sh == 0) break;
+        const int n = min(sh, STEP);
+        const int b = sh - n;   // the step takes s_stack[b .. b + n)
+        const bool two = n <= SPEC;
+        const int per = two ? kSpecSamples : 9;
+
+        // ---- samples: sample j of range g at s_xch[(g * per + j) * 5] ----
+        for (int t = tid; t < per * n; t += NT) {
+            const int g = t / per, j = t - per * g;
+            const RangeRec pr = s_stack[b + g];
+            RangeRec c = pr;
+            int iu = j / 3, iv = j - 3 * (j / 3);
+            if (two) spec_sample_sel(pr, j, c, iu, iv);
+            float v[5] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+            if ((int)(c.obj_depth >> 24) < dlim) forest_sample<PT>(s_obj, s_view, s_cps, slots, CS, c, iu, iv, v);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) s_xch[t * 5 + k] = v[k];
+        }
+        __syncthreads();   // samples written
+
+        // ---- decisions: lane c decides candidate c: range c, or on a
+        // two-level step candidate d = c % 5 (0 the range, 1..4 its possible
+        // children) of range g = c / 5 ----
+        const int g = two ? tid / 5 : tid;
+        const int d = two ? tid - 5 * g : 0;
+        const bool valid = g < n;
+        const RangeRec pr = s_stack[b + min(g, n - 1)];
+        RangeRec rec;
+        spec_range_sel(pr, d, rec);
+        const bool live = valid && (int)(rec.obj_depth >> 24) < dlim;
+        int f = 0;
+        {
+            float ex[9], ey[9], ez[9], sx[9], sy[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int j = two ? spec_index(d, k / 3, k % 3) : k;
+                const int o = live ? (g * per + j) * 5 : 0;
+                ex[k] = s_xch[o]; ey[k] = s_xch[o + 1]; ez[k] = s_xch[o + 2];
+                sx[k] = s_xch[o + 3]; sy[k] = s_xch[o + 4];
+            }
+            if (live) {
+                const ForestObj& ob = s_obj[rec.obj_depth & 0xffffffu];
+                f = bound_decide(ex, ey, ez, sx, sy, rec.umin, rec.vmin, rec.umax, rec.vmax, s_view[ob.view], prm,
+                                 ob.split_limit);
+            }
+        }
+        int flags = f;
+        if (two) {
+            if (valid && d == 0) s_pf[g] = f;
+            __syncthreads();   // the ranges' decisions
+            const int pf = valid ? s_pf[g] : 0;
+            const bool psplit = (pf & 2) != 0, udir = (pf & 4) != 0;
+            flags = 0;
+            if (valid) {
+                if (d == 0) flags = pf == MP_DRAW ? MP_DRAW : 0;
+                else if (d <= 2) flags = (psplit && udir) ? f : 0;
+                else flags = (psplit && !udir) ? f : 0;
+                if (d == 0) nbounds += psplit ? 3u : 1u;
+            }
+        } else if (valid) {
+            nbounds += 1u;
+        }
+        if (!valid) flags = 0;
+
+        // ---- compaction: children onto the stack, DRAW ranges to the buffer ----
+        const uint64_t sb = __ballot((flags & 2) != 0);
+        const uint64_t db = __ballot((flags & 1) != 0);
+        if (lane == 0) {
+            s_wcnt[wave][0] = __popcll(sb);
+            s_wcnt[wave][1] = __popcll(db);
+        }
+        __syncthreads();   // counts posted; every read of the popped ranges done
+        int so = 0, dof = 0, ns = 0, nd = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const int a = s_wcnt[w][0], c = s_wcnt[w][1];
+            so += w < wave ? a : 0;
+            dof += w < wave ? c : 0;
+            ns += a;
+            nd += c;
+        }
+        if (b + 2 * ns > cap) {
+            if (tid == 0) atomicOr(&fc->overflow, 2u);
+            ok = false;
+            break;
+        }
+        if (flags & 2) {
+            const int rk = so + (int)lanes_below(sb);
+            RangeRec c0, c1;
+            split_children(rec, flags, c0, c1);
+            s_stack[b + 2 * rk] = c0;
+            s_stack[b + 2 * rk + 1] = c1;
+        }
+        sh = b + 2 * ns;
+        maxsh = max(maxsh, sh);
+        if (dfill + nd > DBUF) {
+            ok = forest_flush<NT>(objs, cb, fc, s_dbuf, dfill, &s_base);
+            dfill = 0;
+            if (!ok) break;
+        }
+        if (flags & 1) s_dbuf[dfill + dof + (int)lanes_below(db)] = rec;
+        dfill += nd;
+    }
+    __syncthreads();   // the last DRAW buffer writes
+    if (ok && dfill > 0) forest_flush<NT>(objs, cb, fc, s_dbuf, dfill, &s_base);
+    uint32_t nb = nbounds;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nb += (uint32_t)__shfl_xor((int)nb, o);
+    if (lane == 0) atomicAdd(&fc->bounds, (unsigned long long)nb);
+    if (tid == 0) atomicMax(&fc->max_stack, (uint32_t)maxsh);
+}
+
+size_t split_forest_lds(int stack_cap)
+{
+    return (size_t)stack_cap * sizeof(RangeRec);
+}
+
+int split_forest_occupancy(size_t lds_bytes)
+{
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_split_forest<kForestThreads, 2>, kForestThreads,
+                                                     lds_bytes) != hipSuccess)
+        blocks = 1;
+    return blocks < 1 ? 1 : blocks;
+}
+
+bool split_forest_fits(int n_objs, int n_views)
+{
+    return n_objs <= kForestTables && n_views <= kForestTables;
+}
+
+hipError_t launch_split_forest(const DevObject* objs, const DevView* views, const DevParams& prm,
+                               const SplitQueues& q, const CanonBuffers& cb, FrameCounters* fc, int patch_types,
+                               int n_objs, int n_views, hipStream_t s)
+{
+    if (!split_forest_fits(n_objs, n_views)) return hipErrorInvalidValue;
+    const size_t lds = split_forest_lds(q.stack_cap);
+    if (patch_types == 1)
+        hipLaunchKernelGGL((k_split_forest<kForestThreads, 0>), dim3(q.n_groups), dim3(kForestThreads), lds, s,
+                           objs, views, prm, q, cb, fc, n_objs, n_views);
+    else if (patch_types == 2)
+        hipLaunchKernelGGL((k_split_forest<kForestThreads, 1>), dim3(q.n_groups), dim3(kForestThreads), lds, s,
+                           objs, views, prm, q, cb, fc, n_objs, n_views);
+    else
+        hipLaunchKernelGGL((k_split_forest<kForestThreads, 2>), dim3(q.n_groups), dim3(kForestThreads), lds, s,
+                           objs, views, prm, q, cb, fc, n_objs, n_views);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
 // BREADTH / BOUNDED: one level (batch) = bound + block scan + stable move
 
 constexpr int SEG = 256;
