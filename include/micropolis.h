@@ -256,7 +256,11 @@ int mp_replay(mp_context* ctx, int32_t n, int32_t use_graph, double* ms_per_fram
  * bound_n_split_method and work-group sizes). */
 typedef struct mp_tuning {
     int32_t split_engine;    /* 0: persistent LDS-stack split, one launch;
-                                1: one launch per split level + persistent tail */
+                                1: one launch per split level + persistent tail;
+                                2: forest split (default): a work-group splits
+                                the whole trees of its roots with all its lanes
+                                and, for frames of at most 4096 patches, ranks
+                                their DRAW ranges itself (one order launch) */
     int32_t work_sharing;    /* persistent split: deep stacks spill to idle groups */
     int32_t raster_per_cu;   /* raster work-groups per CU (0 = occupancy limit) */
     int32_t raster_chunk;    /* raster range schedule: 0 static stride (no
@@ -280,9 +284,9 @@ int mp_set_tuning(mp_context* ctx, const mp_tuning* tuning);
 /* Autotuner: replay the last finished LOCAL/BALANCED frame or batch under
  * candidate settings, `replays` graph replays each (best of two rounds):
  * first the raster schedule (work-groups per CU x static / dynamic ranges),
- * then the split (persistent with / without work sharing or a step budget,
- * or a number of level launches before the persistent tail); keep the
- * fastest. */
+ * then the split (the forest split, persistent with / without work sharing
+ * or a step budget, or a number of level launches before the persistent
+ * tail); keep the fastest. */
 int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen);
 
 /* Replay the last LOCAL/BALANCED frame n times with HIP events between its

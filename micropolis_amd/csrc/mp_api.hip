@@ -137,6 +137,10 @@ struct mp_context {
     // canonical lists
     RangeRec* d_draw = nullptr; RangeRec* d_bucket = nullptr; RangeRec* d_canon = nullptr;
     unsigned long long* d_sort_keys = nullptr; uint32_t* d_sort_vals = nullptr;   // 2 x list_cap each
+    // ranked forest frames: per-group output regions (kForestKeys entries per
+    // group), in-patch ranks, group totals
+    RangeRec* d_forest_out = nullptr; uint32_t* d_rank = nullptr; uint32_t* d_forest_total = nullptr;
+    int forest_out_groups = 0;
     uint32_t list_cap = 0;
     uint32_t* d_counts = nullptr; uint32_t* d_offsets = nullptr; uint32_t* d_cursor = nullptr;
     uint32_t* d_part = nullptr;
@@ -177,9 +181,12 @@ struct mp_context {
     // level-synchronous split engine
     int split_engine = 0;          // 0 persistent LDS-stack engine, 1 level-synchronous,
                                    // 2 forest (a group splits whole trees with all its lanes)
-    int forest_cap = 2048;         // forest split: LDS stack entries per group
+    int forest_cap = 1536;         // forest split: LDS stack entries per group
     int forest_per_cu = 0;         // forest split: groups per CU (0: occupancy)
     int patch_types = 0;           // bit 0: the frame has Bezier objects, bit 1: Gregory ones
+    int forest_rank = 1;           // forest split: rank the DRAW ranges in the groups (0 after a
+                                   // frame whose groups could not hold their ranges)
+    bool frame_ranked = false;     // the frame being launched takes the ranked order
     RangeRec* d_level_a = nullptr; RangeRec* d_level_b = nullptr; uint32_t level_cap = 0;
     uint32_t* d_level_cnt = nullptr; uint32_t* h_level_cnt = nullptr;
     int level_launches = 0;        // levels launched before the persistent tail
@@ -460,9 +467,9 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
     // split trees sooner than more groups sharing it (measured on the torus frame).
     ctx->stack_cap = 64 * (cfg->max_split_depth + 2);
     // tuning knobs (defaults chosen by measurement; mp_autotune picks per
-    // frame): BALANCED frames split fastest level by level (one shared queue
-    // serialises on its counter), LOCAL frames in the persistent kernel
-    ctx->split_engine = cfg->bound_n_split_method == MP_BNS_BALANCED ? 1 : 0;
+    // frame): LOCAL and BALANCED frames split fastest in the forest engine
+    // (frames whose tables do not fit it take the persistent kernel)
+    ctx->split_engine = 2;
     if (const char* e = getenv("MP_WORK_SHARING")) ctx->work_sharing = atoi(e) != 0;
     if (const char* e = getenv("MP_SPILL_AT")) ctx->spill_at = atoi(e);
     if (const char* e = getenv("MP_LINGER")) ctx->linger = atoi(e);
@@ -510,6 +517,7 @@ extern "C" void mp_destroy(mp_context* ctx)
     hipFree(ctx->d_seg_a); hipFree(ctx->d_seg_b); hipFree(ctx->d_flags); hipFree(ctx->d_blk);
     hipFree(ctx->d_draw); hipFree(ctx->d_bucket); hipFree(ctx->d_canon);
     hipFree(ctx->d_sort_keys); hipFree(ctx->d_sort_vals);
+    hipFree(ctx->d_forest_out); hipFree(ctx->d_rank); hipFree(ctx->d_forest_total);
     hipFree(ctx->d_counts); hipFree(ctx->d_offsets); hipFree(ctx->d_cursor); hipFree(ctx->d_part);
     hipFree(ctx->d_quad);
     if (ctx->h_ring) hipHostFree(ctx->h_ring);
@@ -739,6 +747,12 @@ static CanonBuffers canon_buffers(mp_context* ctx)
     cb.n_gpids = ctx->n_gpids;
     cb.sort_keys = ctx->d_sort_keys;
     cb.sort_vals = ctx->d_sort_vals;
+    if (ctx->split_engine == 2 && ctx->forest_out_groups >= ctx->split_groups) {
+        cb.forest_out = ctx->d_forest_out;
+        cb.rank = ctx->d_rank;
+        cb.forest_total = ctx->d_forest_total;
+        cb.forest_groups = (uint32_t)ctx->split_groups;
+    }
     return cb;
 }
 
@@ -871,6 +885,16 @@ static int ensure_queues(mp_context* ctx)
         groups = ctx->num_cus * split_persistent_occupancy(split_persistent_lds(ctx->stack_cap), ctx->split_waves);
     }
     ctx->split_groups = groups;
+    if (ctx->split_engine == 2 && groups > ctx->forest_out_groups) {
+        // ranked forest frames: one output region of kForestKeys ranges per group
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        const size_t n = (size_t)groups * kForestKeys;
+        if ((rc = dev_alloc(ctx, &ctx->d_forest_out, n)) || (rc = dev_alloc(ctx, &ctx->d_rank, n)) ||
+            (rc = dev_alloc(ctx, &ctx->d_forest_total, (size_t)groups)))
+            return rc;
+        ctx->forest_out_groups = groups;
+        invalidate_graph(ctx);
+    }
     if (ctx->split_engine == 1 || rebalancing(ctx)) {
         // level lists: sized for the widest level; overflow re-renders bigger
         const uint32_t want = std::max<uint32_t>(ctx->level_cap, std::max<uint32_t>(65536, 8 * ctx->total_roots));
@@ -961,6 +985,10 @@ static SplitQueues split_queues(mp_context* ctx)
 static FrameEnd frame_end(mp_context* ctx)
 {
     FrameEnd fe;
+    if (ctx->frame_ranked) {
+        fe.counts = ctx->d_counts;
+        fe.n_counts = ctx->n_gpids;
+    }
     fe.fc = ctx->d_fc;
     fe.out = ctx->d_fc_out;
     fe.share = ctx->d_share;
@@ -978,8 +1006,12 @@ static FrameEnd frame_end(mp_context* ctx)
 static int launch_back_end(mp_context* ctx, hipEvent_t* ev = nullptr)
 {
     CanonBuffers cb = canon_buffers(ctx);
-    HIPCHK(ctx, launch_canonicalize(ctx->d_objs, ctx->d_views, ctx->n_views, cb, ctx->d_fc,
-                                    ctx->num_cus, ctx->fused_order, ctx->stream));
+    if (ctx->frame_ranked)
+        HIPCHK(ctx, launch_place_ranked(ctx->d_objs, ctx->d_views, ctx->n_views, cb, ctx->d_fc, ctx->num_cus,
+                                        ctx->stream));
+    else
+        HIPCHK(ctx, launch_canonicalize(ctx->d_objs, ctx->d_views, ctx->n_views, cb, ctx->d_fc,
+                                        ctx->num_cus, ctx->fused_order, ctx->stream));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
     if (!ctx->cfg.dummy_render)
         HIPCHK(ctx, launch_raster(ctx->cfg.reyes_patch_size, ctx->d_canon, ctx->d_objs, ctx->d_views,
@@ -1021,6 +1053,8 @@ static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
     const bool need_init = !frame_ready(ctx);
     ctx->frame_inits = need_init;   // read by a capture of this frame
     ctx->armed = false;   // until this frame's end
+    ctx->frame_ranked = ctx->total_roots && ctx->split_engine == 2 && ctx->forest_rank &&
+                        forest_rank_fits(ctx->n_gpids) && ctx->forest_out_groups >= ctx->split_groups;
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
     // the first split launch resets the frame counters
     if (!ctx->total_roots) HIPCHK(ctx, launch_reset(ctx->d_fc, ctx->d_share, ctx->stream));
@@ -1039,7 +1073,8 @@ static int launch_device_frame(mp_context* ctx, hipEvent_t* ev = nullptr)
             ctx->init_sig = sig;
         }
         HIPCHK(ctx, launch_split_forest(ctx->d_objs, ctx->d_views, ctx->prm, q, canon_buffers(ctx), ctx->d_fc,
-                                        ctx->patch_types, (int)ctx->objects.size(), ctx->n_views, ctx->stream));
+                                        ctx->patch_types, (int)ctx->objects.size(), ctx->n_views,
+                                        ctx->frame_ranked ? 1 : 0, ctx->stream));
     } else if (ctx->total_roots && rebalancing(ctx)) {
         q.step_budget = ctx->split_budget;
         q.left = ctx->d_level_a;
@@ -1325,6 +1360,7 @@ static int execute_frame(mp_context* ctx)
 {
     int rc;
     ctx->frame_valid = false;
+    ctx->frame_ranked = false;   // set by a device frame that takes the ranked order
     invalidate_graph(ctx);  // a captured graph bakes the previous frame's launch sizes
     if ((rc = ensure_views(ctx))) return rc;
     if ((rc = upload_tables(ctx))) return rc;
@@ -1350,7 +1386,14 @@ static int execute_frame(mp_context* ctx)
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
             const uint32_t ovf = ctx->h_fc->overflow;
             if (ovf & 4u) return fail(ctx, MP_E_HIP, "split work sharing timed out");
-            if (!(ovf & 11u)) break;
+            if (!(ovf & 43u)) break;
+            if (ovf & 32u) {
+                // a forest group held more DRAW ranges than it can rank:
+                // the frame takes the unranked order
+                ctx->forest_rank = 0;
+                ctx->counts_dirty = true;
+                if ((rc = clear_stale_counts(ctx))) return rc;
+            }
             if (ovf & 8u) {
                 ctx->level_cap *= 4;   // a split level outgrew its list
                 HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1841,11 +1884,18 @@ static int apply_tuning(mp_context* ctx, const mp_tuning& t)
     if (!ctx->frame_valid || !ctx->device_driven) return MP_OK;
     int rc;
     if ((rc = ensure_queues(ctx))) return rc;
-    if ((rc = clear_stale_counts(ctx))) return rc;
-    ctx->counts_dirty = true;
-    if ((rc = launch_device_frame(ctx))) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc_out, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int attempt = 0;; ++attempt) {
+        if ((rc = clear_stale_counts(ctx))) return rc;
+        ctx->counts_dirty = true;
+        if ((rc = launch_device_frame(ctx))) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->h_fc, ctx->d_fc_out, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->h_fc->overflow == 32u && attempt == 0) {
+            ctx->forest_rank = 0;   // the forest groups cannot rank this frame
+            continue;
+        }
+        break;
+    }
     if (ctx->h_fc->overflow) return fail(ctx, MP_E_STATE, "frame overflowed under this tuning");
     ctx->counts_dirty = false;
     if (ctx->split_engine == 1 && (rc = update_level_launches(ctx, nullptr))) return rc;

@@ -61,7 +61,19 @@ struct CanonBuffers {
     // and 2 x cap slot indices (ping-pong)
     unsigned long long* sort_keys;
     uint32_t* sort_vals;
+    // forest split with in-group ranks (null where the path is not used):
+    // group g's DRAW ranges at forest_out[g * kForestKeys ..], their ranks
+    // among their patches' DRAW ranges at the same indices of rank, their
+    // number at forest_total[g] (forest_groups groups)
+    RangeRec* forest_out = nullptr;
+    uint32_t* rank = nullptr;
+    uint32_t* forest_total = nullptr;
+    uint32_t forest_groups = 0;
 };
+
+// DRAW ranges one forest group can rank (its LDS key store and its output
+// region); a group with more flags overflow bit 32.
+constexpr int kForestKeys = 1536;
 
 hipError_t launch_reset(FrameCounters* fc, ShareState* share, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);
@@ -115,6 +127,9 @@ struct FrameEnd {
     uint32_t* left_cnt = nullptr;     // reset to 0
     int32_t* shared_cnt = nullptr;    // reset to shared_total
     int32_t shared_total = 0;
+    uint32_t* counts = nullptr;       // per-patch DRAW counts cleared (n_counts), after a
+    uint32_t n_counts = 0;            // ranked frame that wrote them with plain stores
+    bool resolve_clears = false;      // set by launch_resolve: its threads clear them
 };
 hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           const RangeRec* canon, const DevObject* objs, const float* quad_store,
@@ -155,9 +170,21 @@ int split_persistent_occupancy(size_t lds_bytes, int waves);   // waves: 2 or 3 
 // q.stack_cap entries (at least kForestMinStack).
 // patch_types: bit 0 some object has Bezier patches, bit 1 some has Gregory
 // patches (the kernel build without the other evaluation is launched)
+// ranked: the groups also rank their DRAW ranges inside each patch
+// (cb.rank) and write every root's DRAW count with a plain store, for
+// launch_place_ranked; a group with more DRAW ranges than its LDS key store
+// flags overflow bit 32 (the frame must then run unranked).  Unranked: the
+// counts are added atomically (zero on entry) for launch_canonicalize.
 hipError_t launch_split_forest(const DevObject* objs, const DevView* views, const DevParams& prm,
                                const SplitQueues& q, const CanonBuffers& cb, FrameCounters* fc, int patch_types,
-                               int n_objs, int n_views, hipStream_t s);
+                               int n_objs, int n_views, int ranked, hipStream_t s);
+// whether a frame of n_gpids patches can take the ranked forest order
+bool forest_rank_fits(uint32_t n_gpids);
+// The canonical order of a ranked forest frame in one launch: every
+// work-group scans the per-patch counts and places its share of the draw
+// list at offset(patch) + rank (no bucket pass, no rank kernel).
+hipError_t launch_place_ranked(const DevObject* objs, DevView* views, int n_views, const CanonBuffers& cb,
+                               FrameCounters* fc, int num_cus, hipStream_t s);
 // whether a frame's object and view tables fit the forest split's LDS copy
 bool split_forest_fits(int n_objs, int n_views);
 int split_forest_occupancy(size_t lds_bytes);

@@ -28,6 +28,10 @@ namespace mp {
 // ------------------------------------------------------------------------
 // small utilities
 
+// Per-patch count scans done by every work-group of a launch into LDS
+// (k_scan_scatter, k_place_ranked): frames of at most this many patches.
+constexpr uint32_t kFusedScan = 4096;
+
 // Start-of-frame counters (one thread).
 MP_DEV void reset_counters(FrameCounters* fc, ShareState* share)
 {
@@ -59,6 +63,7 @@ __global__ void k_reset(FrameCounters* fc, ShareState* share)
 MP_DEV void end_frame(const FrameEnd& fe)
 {
     if (!fe.fc) return;
+    for (uint32_t i = 0; i < fe.n_counts && !fe.resolve_clears; ++i) fe.counts[i] = 0u;
     *fe.out = *fe.fc;
     reset_counters(fe.fc, fe.share);
     if (fe.left_cnt) *fe.left_cnt = 0u;
@@ -845,7 +850,7 @@ hipError_t launch_split_persistent(const DevObject* objs, const DevView* views,
 // the kernel: the code of the whole kernel stays inside the instruction
 // cache), with the object and view tables read per lane; PT selects the
 // patch types compiled in (0 Bezier, 1 Gregory, 2 both).
-constexpr int kForestThreads = 512;
+constexpr int kForestThreads = 768;
 
 // spec_sample and spec_range with every field chosen by selects (a record
 // whose fields are stored by a runtime index goes through scratch memory)
@@ -894,7 +899,40 @@ struct ForestView {
 };
 // Control points of a group's roots staged in LDS: root slot k (the group's
 // k-th root, global patch index wid + k * G) at k * stride float4.
-constexpr int kForestCps = 2048;   // float4
+constexpr int kForestCps = 1024;   // float4
+// Ranked frames: roots per group (more: the frame runs unranked); the DRAW
+// ranges a group can rank are kForestKeys (its LDS key store and output region).
+constexpr int kForestRankRoots = 64;
+constexpr int kRankBins = 4096;   // columns of the column-sorted ranking
+
+// Exclusive scan of in[0 .. n) into out[0 .. n) (LDS) by the NT threads of
+// a work-group, each taking a contiguous chunk; s_wsum holds NT / 64 words.
+// Ends with a barrier.
+template <int NT>
+MP_DEV void lds_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* s_wsum)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t per = (n + NT - 1u) / NT;
+    const uint32_t b0 = min(tid * per, n), b1 = min(b0 + per, n);
+    uint32_t sum = 0;
+    for (uint32_t i = b0; i < b1; ++i) sum += in[i];
+    uint32_t v = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o);
+        if ((int)lane >= o) v += y;
+    }
+    if (lane == 63) s_wsum[wave] = v;
+    __syncthreads();
+    uint32_t run = v - sum;
+    for (uint32_t w = 0; w < wave; ++w) run += s_wsum[w];
+    for (uint32_t i = b0; i < b1; ++i) {
+        const uint32_t x = in[i];
+        out[i] = run;
+        run += x;
+    }
+    __syncthreads();
+}
 
 // bound() sample (iu, iv) of rec: eye-space x, y, z and screen x, y; the
 // range's control points from the group's LDS copy when its root has a slot
@@ -962,16 +1000,20 @@ template <int NT, int PT>
 __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict__ objs,
                                                      const DevView* __restrict__ views, DevParams prm,
                                                      SplitQueues q, CanonBuffers cb, FrameCounters* fc,
-                                                     int n_objs, int n_views)
+                                                     int n_objs, int n_views, int ranked)
 {
     static_assert(NT % 64 == 0 && NT >= 256, "whole waves");
     constexpr int NW = NT / 64;
-    constexpr int STEP = NT / 2;                   // ranges per step
-    constexpr int SPEC = NT / kSpecSamples;        // ranges of a two-level step
+    constexpr int STEP = NT / 2 < 256 ? NT / 2 : 256;   // ranges per step
+    // ranges of a two-level step: five candidate decisions each on one lane
+    // pass (a wider two-level step costs more sample passes than the level
+    // it saves)
+    constexpr int SPEC = NT / 5 < 128 ? NT / 5 : 128;
+    constexpr int XCH = (kSpecSamples * SPEC > 9 * STEP ? kSpecSamples * SPEC : 9 * STEP) * 5;
     static_assert(5 * SPEC <= NT, "two-level candidates");
     constexpr int DBUF = NT;                       // buffered DRAW ranges
     extern __shared__ RangeRec s_stack[];          // q.stack_cap entries
-    __shared__ float s_xch[9 * STEP * 5];          // the step's samples, 5 floats each
+    __shared__ __align__(16) float s_xch[XCH];    // the step's samples, 5 floats each
     __shared__ RangeRec s_dbuf[DBUF];              // DRAW ranges not yet appended
     __shared__ int s_wcnt[NW][2];                  // per wave: split / draw lanes
     __shared__ int s_pf[SPEC + 1];                 // two-level steps: the ranges' decisions
@@ -979,7 +1021,16 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
     __shared__ ForestObj s_obj[kForestTables];
     __shared__ ForestView s_view[kForestTables];
     __shared__ float4 s_cps[kForestCps];
-    static_assert(kSpecSamples * SPEC <= 9 * STEP, "sample buffer");
+    // ranked frames: every DRAW range of the group as (root slot << 48 |
+    // canonical key), in emission order (its record is at the same index of
+    // the group's output region)
+    __shared__ unsigned long long s_dk[kForestKeys];
+    __shared__ uint32_t s_hist[kForestRankRoots];
+    __shared__ uint32_t s_wcnt_scan[2 * kForestRankRoots];
+    __shared__ uint32_t s_wsum[NW];
+    static_assert(kForestRankRoots == 64, "one wave scans the bucket sizes");
+    static_assert(kForestKeys * 12 + kRankBins * 8 <= (int)sizeof(s_xch), "rank lists fit the sample buffer");
+    static_assert(5 * SPEC <= NT, "one lane per two-level candidate");
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wid = blockIdx.x;
@@ -987,9 +1038,11 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
     const int dlim = prm.max_split_depth - 1;      // bound() culls ranges this deep
     const RangeRec* myq = q.queue + (int64_t)wid * q.stride;
     int qcnt = q.queue_cnt[wid];
-    int sh = 0, dfill = 0, maxsh = 0;
+    const int nroots = qcnt;
+    int sh = 0, dfill = 0, maxsh = 0, kfill = 0;
     uint32_t nbounds = 0;
     bool ok = true;
+    bool keys_ok = ranked != 0 && nroots <= kForestRankRoots;
 
     // tables and the roots' control points to LDS
     for (int i = tid; i < n_objs; i += NT) {
@@ -1131,6 +1184,19 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
         }
         sh = b + 2 * ns;
         maxsh = max(maxsh, sh);
+        if (ranked) {
+            // the group's own output region: no shared counter
+            if (kfill + nd > kForestKeys) keys_ok = false;
+            if ((flags & 1) && keys_ok) {
+                const int at = kfill + dof + (int)lanes_below(db);
+                const ForestObj& ob = s_obj[rec.obj_depth & 0xffffffu];
+                const uint32_t slot = (ob.gpid_base + (uint32_t)rec.pid - (uint32_t)wid) / (uint32_t)gridDim.x;
+                s_dk[at] = ((unsigned long long)slot << 48) | canonical_key(rec);
+                cb.forest_out[(size_t)wid * kForestKeys + at] = rec;
+            }
+            if (keys_ok) kfill += nd;
+            continue;
+        }
         if (dfill + nd > DBUF) {
             ok = forest_flush<NT>(objs, cb, fc, s_dbuf, dfill, &s_base);
             dfill = 0;
@@ -1140,7 +1206,124 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
         dfill += nd;
     }
     __syncthreads();   // the last DRAW buffer writes
-    if (ok && dfill > 0) forest_flush<NT>(objs, cb, fc, s_dbuf, dfill, &s_base);
+    if (ok && dfill > 0) ok = forest_flush<NT>(objs, cb, fc, s_dbuf, dfill, &s_base);
+    if (ranked && ok) {
+        if (!keys_ok) {
+            if (tid == 0) atomicOr(&fc->overflow, 32u);
+        } else {
+            // rank of each DRAW range among its patch's (the group owns the
+            // patch's whole tree), and every root's DRAW count.  Keys are
+            // (slot, u, v): with g = the fewest trailing zero bits of any u
+            // (the finest split along u), (slot, u >> g) numbers the group's
+            // columns in key order; when there are at most kRankBins of them
+            // a counting sort by column leaves each key's rank as its
+            // column's offset plus the smaller keys of its column (a few).
+            // Otherwise the keys are bucketed by slot and each counts the
+            // smaller keys of its bucket.
+            unsigned long long* s_bk = reinterpret_cast<unsigned long long*>(s_xch);
+            uint32_t* s_bg = reinterpret_cast<uint32_t*>(s_bk + kForestKeys);
+            uint32_t* s_col = s_bg + kForestKeys;          // kRankBins column offsets
+            uint32_t* s_cur = s_col + kRankBins;           // kRankBins cursors
+            if (tid == 0) s_base = 24u;
+            __syncthreads();
+            {
+                uint32_t tz = 24u;
+                for (int i = tid; i < kfill; i += NT) {
+                    const uint32_t u = (uint32_t)(s_dk[i] >> 24) & 0xffffffu;
+                    tz = min(tz, u ? (uint32_t)__builtin_ctz(u) : 24u);
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) tz = min(tz, (uint32_t)__shfl_xor((int)tz, o));
+                if (lane == 0) atomicMin(&s_base, tz);
+            }
+            __syncthreads();
+            const uint32_t gsh = s_base;                       // column = u >> gsh
+            const uint32_t ncol = 1u << (24u - gsh);           // columns per root
+            const bool columns = (uint64_t)nroots * ncol <= (uint64_t)kRankBins;
+            if (columns) {
+                const uint32_t nb = (uint32_t)nroots * ncol;
+                for (uint32_t i = tid; i < nb; i += NT) s_cur[i] = 0u;
+                __syncthreads();
+                for (int i = tid; i < kfill; i += NT) {
+                    const unsigned long long K = s_dk[i];
+                    const uint32_t bin = (uint32_t)(K >> 48) * ncol + (((uint32_t)(K >> 24) & 0xffffffu) >> gsh);
+                    atomicAdd(&s_cur[bin], 1u);
+                }
+                __syncthreads();
+                lds_exclusive_scan<NT>(s_cur, s_col, nb, s_wsum);   // s_col = offsets; s_cur keeps the sizes
+                for (uint32_t i = tid; i < nb; i += NT) s_cur[i] = s_col[i];
+                __syncthreads();
+                for (int i = tid; i < kfill; i += NT) {
+                    const unsigned long long K = s_dk[i];
+                    const uint32_t bin = (uint32_t)(K >> 48) * ncol + (((uint32_t)(K >> 24) & 0xffffffu) >> gsh);
+                    const uint32_t at = atomicAdd(&s_cur[bin], 1u);
+                    s_bk[at] = K;
+                    s_bg[at] = (uint32_t)i;
+                }
+                __syncthreads();
+                for (int p = tid; p < kfill; p += NT) {
+                    const unsigned long long K = s_bk[p];
+                    const uint32_t sl = (uint32_t)(K >> 48);
+                    const uint32_t bin = sl * ncol + (((uint32_t)(K >> 24) & 0xffffffu) >> gsh);
+                    // s_cur[bin] is now the column's end
+                    const uint32_t j0 = s_col[bin], j1 = s_cur[bin];
+                    uint32_t r = j0 - s_col[sl * ncol];
+                    for (uint32_t j = j0; j < j1; ++j) r += s_bk[j] < K ? 1u : 0u;
+                    cb.rank[(size_t)wid * kForestKeys + s_bg[p]] = r;
+                }
+                for (int k = tid; k < nroots; k += NT) {
+                    const uint32_t a = s_col[(uint32_t)k * ncol];
+                    const uint32_t e = (uint32_t)(k + 1) < (uint32_t)nroots ? s_col[(uint32_t)(k + 1) * ncol]
+                                                                          : (uint32_t)kfill;
+                    cb.counts[(uint32_t)wid + (uint32_t)k * gridDim.x] = e - a;
+                }
+            } else {
+                uint32_t* s_bs = s_wcnt_scan;
+                for (int i = tid; i < kForestRankRoots; i += NT) s_hist[i] = 0u;
+                __syncthreads();
+                for (int i = tid; i < kfill; i += NT) atomicAdd(&s_hist[(uint32_t)(s_dk[i] >> 48)], 1u);
+                __syncthreads();
+                if (tid < 64) {
+                    // exclusive scan of the kForestRankRoots (64) bucket sizes
+                    const uint32_t x = s_hist[tid];
+                    uint32_t v = x;
+#pragma unroll
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const uint32_t y = __shfl_up(v, o);
+                        if (lane >= o) v += y;
+                    }
+                    s_bs[tid] = v - x;          // bucket start
+                    s_bs[64 + tid] = v - x;     // fill cursor
+                }
+                __syncthreads();
+                for (int i = tid; i < kfill; i += NT) {
+                    const unsigned long long K = s_dk[i];
+                    const uint32_t at = atomicAdd(&s_bs[64 + (uint32_t)(K >> 48)], 1u);
+                    s_bk[at] = K;
+                    s_bg[at] = (uint32_t)i;
+                }
+                __syncthreads();
+                for (int p = tid; p < kfill; p += NT) {
+                    const unsigned long long K = s_bk[p];
+                    const uint32_t sl = (uint32_t)(K >> 48);
+                    const int j0 = (int)s_bs[sl], j1 = j0 + (int)s_hist[sl];
+                    uint32_t r = 0;
+                    int j = j0;
+                    for (; j + 8 <= j1; j += 8) {
+                        uint32_t c = 0;
+#pragma unroll
+                        for (int q8 = 0; q8 < 8; ++q8) c += s_bk[j + q8] < K ? 1u : 0u;
+                        r += c;
+                    }
+                    for (; j < j1; ++j) r += s_bk[j] < K ? 1u : 0u;
+                    cb.rank[(size_t)wid * kForestKeys + s_bg[p]] = r;
+                }
+                for (int k = tid; k < nroots; k += NT)
+                    cb.counts[(uint32_t)wid + (uint32_t)k * gridDim.x] = s_hist[k];
+            }
+            if (tid == 0) cb.forest_total[wid] = (uint32_t)kfill;
+        }
+    }
     uint32_t nb = nbounds;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) nb += (uint32_t)__shfl_xor((int)nb, o);
@@ -1167,21 +1350,26 @@ bool split_forest_fits(int n_objs, int n_views)
     return n_objs <= kForestTables && n_views <= kForestTables;
 }
 
+bool forest_rank_fits(uint32_t n_gpids)
+{
+    return n_gpids <= kFusedScan;
+}
+
 hipError_t launch_split_forest(const DevObject* objs, const DevView* views, const DevParams& prm,
                                const SplitQueues& q, const CanonBuffers& cb, FrameCounters* fc, int patch_types,
-                               int n_objs, int n_views, hipStream_t s)
+                               int n_objs, int n_views, int ranked, hipStream_t s)
 {
     if (!split_forest_fits(n_objs, n_views)) return hipErrorInvalidValue;
     const size_t lds = split_forest_lds(q.stack_cap);
     if (patch_types == 1)
         hipLaunchKernelGGL((k_split_forest<kForestThreads, 0>), dim3(q.n_groups), dim3(kForestThreads), lds, s,
-                           objs, views, prm, q, cb, fc, n_objs, n_views);
+                           objs, views, prm, q, cb, fc, n_objs, n_views, ranked);
     else if (patch_types == 2)
         hipLaunchKernelGGL((k_split_forest<kForestThreads, 1>), dim3(q.n_groups), dim3(kForestThreads), lds, s,
-                           objs, views, prm, q, cb, fc, n_objs, n_views);
+                           objs, views, prm, q, cb, fc, n_objs, n_views, ranked);
     else
         hipLaunchKernelGGL((k_split_forest<kForestThreads, 2>), dim3(q.n_groups), dim3(kForestThreads), lds, s,
-                           objs, views, prm, q, cb, fc, n_objs, n_views);
+                           objs, views, prm, q, cb, fc, n_objs, n_views, ranked);
     return hipGetLastError();
 }
 
@@ -1232,6 +1420,14 @@ hipError_t launch_bound_segment(const DevObject* objs, const DevView* views,
     hipLaunchKernelGGL(k_bound_segment, dim3(segment_blocks(n)), dim3(SEG), 0, s, objs, views,
                        prm, in, pad, n, flags, blk_counts, fc);
     return hipGetLastError();
+}
+
+MP_DEV uint32_t block_exclusive_scan_1024(uint32_t x, uint32_t* lds_waves, uint32_t& total);
+MP_DEV uint32_t block_sum_1024(uint32_t x, uint32_t* lds_waves)
+{
+    uint32_t total;
+    block_exclusive_scan_1024(x, lds_waves, total);
+    return total;
 }
 
 // exclusive block scan of 1024 values held one per thread; returns the total
@@ -1485,7 +1681,6 @@ __global__ __launch_bounds__(256) void k_scatter(const DevObject* __restrict__ o
 // of the draw list from there, as k_scatter does: one launch instead of a scan
 // launch and a scatter launch.  It relies on the counts being those of this
 // frame only and on zero cursors: k_rank clears both after every scatter.
-constexpr uint32_t kFusedScan = 4096;
 
 __global__ __launch_bounds__(1024) void k_scan_scatter(const DevObject* __restrict__ objs, CanonBuffers cb,
                                                        FrameCounters* fc, DevView* views, int n_views)
@@ -1555,6 +1750,98 @@ __global__ __launch_bounds__(1024) void k_scan_scatter(const DevObject* __restri
             cb.span[slot] = make_uint2(s_off[g], min(s_off[g + 1], cb.cap));
         }
     }
+}
+
+// Canonical order of a ranked forest frame (k_split_forest with ranked):
+// work-group g places forest group g's DRAW ranges at their patch's offset +
+// their rank inside the patch, the offsets from its own scan of the
+// per-patch DRAW counts (kFusedScan at most, 16 per thread in LDS).
+constexpr int kPlaceThreads = 256;
+
+__global__ __launch_bounds__(kPlaceThreads) void k_place_ranked(const DevObject* __restrict__ objs,
+                                                                CanonBuffers cb, FrameCounters* fc, DevView* views,
+                                                                int n_views)
+{
+    constexpr uint32_t PER = kFusedScan / kPlaceThreads;   // counts per thread
+    __shared__ uint32_t s_off[kFusedScan + 1];
+    __shared__ uint32_t s_w[2][kPlaceThreads / 64];
+    const uint32_t n = cb.n_gpids;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t g = blockIdx.x;
+    const uint32_t cnt = min(cb.forest_total[g], (uint32_t)kForestKeys);
+    uint32_t c[PER], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t i = t * PER + k;
+        c[k] = i < n ? cb.counts[i] : 0u;
+        sum += c[k];
+    }
+    // block exclusive scan of the per-thread sums (and, for work-group 0,
+    // the sum of the forest groups' totals)
+    uint32_t emitted = 0;
+    if (g == 0)
+        for (uint32_t q = t; q < cb.forest_groups; q += kPlaceThreads) emitted += cb.forest_total[q];
+    uint32_t v = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o);
+        if ((int)lane >= o) v += y;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) emitted += (uint32_t)__shfl_xor((int)emitted, o);
+    if (lane == 63) s_w[0][wave] = v;
+    if (lane == 0) s_w[1][wave] = emitted;
+    __syncthreads();
+    uint32_t run = v - sum, counted = 0, emitted_all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kPlaceThreads / 64; ++w) {
+        run += w < wave ? s_w[0][w] : 0u;
+        counted += s_w[0][w];
+        emitted_all += s_w[1][w];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t i = t * PER + k;
+        if (i < n) s_off[i] = run;
+        run += c[k];
+    }
+    if (t == 0) s_off[n] = counted;
+    __syncthreads();
+    // a split that overflowed a stack or a group's key store left ranks and
+    // counts that do not match its output, and a frame with more ranges than
+    // the canonical list holds must grow it: nothing is placed (the host
+    // renders the frame again)
+    const bool whole = (fc->overflow & 34u) == 0u && counted <= cb.cap;
+    if (g == 0) {
+        if (t == 0) {
+            fc->total_ranges = whole ? counted : 0u;
+            fc->draw_count = whole ? emitted_all : 0u;
+            if (counted > cb.cap) atomicOr(&fc->overflow, 1u);
+        }
+        for (int vv = (int)t; vv < n_views; vv += kPlaceThreads) {
+            const uint32_t b = whole ? s_off[views[vv].first_gpid] : 0u;
+            views[vv].range_base = b;
+            views[vv].range_count = whole ? s_off[views[vv].gpid_end] - b : 0u;
+        }
+    }
+    if (!whole) return;
+    for (uint32_t i = t; i < cnt; i += kPlaceThreads) {
+        const size_t e = (size_t)g * kForestKeys + i;
+        const RangeRec r = cb.forest_out[e];
+        const uint32_t slot = s_off[gpid_of(objs, r)] + cb.rank[e];
+        if (slot < cb.cap) cb.canon[slot] = r;
+    }
+}
+
+hipError_t launch_place_ranked(const DevObject* objs, DevView* views, int n_views, const CanonBuffers& cb,
+                               FrameCounters* fc, int num_cus, hipStream_t s)
+{
+    if (cb.n_gpids > kFusedScan || !cb.rank || !cb.forest_out || !cb.forest_total || !cb.forest_groups)
+        return hipErrorInvalidValue;
+    (void)num_cus;
+    hipLaunchKernelGGL(k_place_ranked, dim3(cb.forest_groups), dim3(kPlaceThreads), 0, s, objs, cb, fc, views,
+                       n_views);
+    return hipGetLastError();
 }
 
 // Rank of each bucketed range among its patch's ranges (its canonical
@@ -2459,6 +2746,11 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
 {
     __shared__ float s_tab[256];
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) end_frame(fe);   // the resolve reads no counter
+    if (blockIdx.y == 0) {
+        // per-patch counts of a ranked frame (nothing reads them after the order)
+        const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+        for (uint32_t k = i; k < fe.n_counts; k += gridDim.x * 256u) fe.counts[k] = 0u;
+    }
     if (threadIdx.x < 255) s_tab[threadIdx.x] = display_tab[threadIdx.x];
     __syncthreads();
     const DevView& vw = views[blockIdx.y];
@@ -2507,8 +2799,10 @@ hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
     int bx = (max_pixels + 255) / 256;
     if (bx > 65535) bx = 65535;
     if (bx < 1) bx = 1;
+    FrameEnd fr = fe;
+    fr.resolve_clears = true;
     hipLaunchKernelGGL(k_resolve, dim3(bx, n_views), dim3(256), 0, s, views, canon, objs,
-                       quad_store, patch_size * patch_size, write_rgba32, display_tab, fe);
+                       quad_store, patch_size * patch_size, write_rgba32, display_tab, fr);
     return hipGetLastError();
 }
 
