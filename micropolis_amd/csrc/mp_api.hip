@@ -437,7 +437,7 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
         hipGetDeviceProperties(&prop, ctx->device) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_fc, sizeof(FrameCounters)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_fc, kFrameCountersBytes) != hipSuccess ||   // counters + their stripes
         hipMalloc((void**)&ctx->d_fc_out, sizeof(FrameCounters)) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_fc, sizeof(FrameCounters), 0) != hipSuccess ||
         hipMalloc((void**)&ctx->d_totals, 4 * sizeof(uint32_t)) != hipSuccess ||

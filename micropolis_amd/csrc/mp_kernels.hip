@@ -32,9 +32,41 @@ namespace mp {
 // (k_scan_scatter, k_place_ranked): frames of at most this many patches.
 constexpr uint32_t kFusedScan = 4096;
 
+// Word w of the stripe of work-group key (kStripes stripes after the counters)
+MP_DEV unsigned long long* stripe(FrameCounters* fc, uint32_t key, int w)
+{
+    return reinterpret_cast<unsigned long long*>(fc + 1) + (size_t)(key % (uint32_t)kStripes) * kStripeWords + w;
+}
+
+// One whole wave: the stripes added into the counters and cleared.
+MP_DEV void fold_stripes(FrameCounters* fc)
+{
+    const int lane = threadIdx.x & 63;
+    static_assert(kStripes == 64, "one stripe per lane");
+    unsigned long long* st = stripe(fc, (uint32_t)lane, 0);
+    unsigned long long a = st[kStripeAtomics], b = st[kStripeBounds], m = st[kStripeMaxStack];
+    st[kStripeAtomics] = 0ull;
+    st[kStripeBounds] = 0ull;
+    st[kStripeMaxStack] = 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+        const unsigned long long mo = __shfl_xor(m, o);
+        m = mo > m ? mo : m;
+    }
+    if (lane == 0) {
+        fc->atomics += a;
+        fc->bounds += b;
+        fc->max_stack = max(fc->max_stack, (uint32_t)m);
+    }
+}
+
 // Start-of-frame counters (one thread).
 MP_DEV void reset_counters(FrameCounters* fc, ShareState* share)
 {
+    for (int k = 0; k < kStripes; ++k)
+        for (int w = 0; w < kStripeUsed; ++w) *stripe(fc, (uint32_t)k, w) = 0ull;
     if (share) {
         share->head = 0;
         share->tail = 0;
@@ -58,8 +90,8 @@ __global__ void k_reset(FrameCounters* fc, ShareState* share)
     if (threadIdx.x == 0) reset_counters(fc, share);
 }
 
-// One thread: FrameEnd's copy and resets (the frame's kernels before it have
-// finished with the counters).
+// One thread, after fold_stripes: FrameEnd's copy and resets (the frame's
+// kernels before it have finished with the counters).
 MP_DEV void end_frame(const FrameEnd& fe)
 {
     if (!fe.fc) return;
@@ -72,6 +104,8 @@ MP_DEV void end_frame(const FrameEnd& fe)
 
 __global__ void k_frame_end(FrameEnd fe)
 {
+    if (!fe.fc) return;
+    fold_stripes(fe.fc);
     if (threadIdx.x == 0) end_frame(fe);
 }
 
@@ -811,8 +845,8 @@ __global__ __launch_bounds__(64 * kSplitWaves) __attribute__((amdgpu_waves_per_e
     __syncthreads();   // release the sample waves
     if (lane == 0) {
         if (sharing && active) atomicSub(&q.share->active, 1u);
-        atomicAdd(&fc->bounds, (unsigned long long)nbounds);
-        atomicMax(&fc->max_stack, maxsh);
+        atomicAdd(stripe(fc, (uint32_t)wid, kStripeBounds), (unsigned long long)nbounds);
+        atomicMax(stripe(fc, (uint32_t)wid, kStripeMaxStack), (unsigned long long)maxsh);
     }
 }
 
@@ -936,12 +970,10 @@ MP_DEV void lds_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, ui
 
 // bound() sample (iu, iv) of rec: eye-space x, y, z and screen x, y; the
 // range's control points from the group's LDS copy when its root has a slot
-template <int PT>
-MP_DEV void forest_sample(const ForestObj* s_obj, const ForestView* s_view, const float4* s_cps, int slots,
-                          int stride, const RangeRec& rec, int iu, int iv, float (&v)[5])
+template <int PT, typename O, typename V>
+MP_DEV void forest_sample(const O& ob, const V& P, const float4* s_cps, int slots, int stride, const RangeRec& rec,
+                          int iu, int iv, float (&v)[5])
 {
-    const ForestObj& ob = s_obj[rec.obj_depth & 0xffffffu];
-    const ForestView& P = s_view[ob.view];
     const uint32_t k = (ob.gpid_base + (uint32_t)rec.pid - (uint32_t)blockIdx.x) / gridDim.x;
     const bool cached = k < (uint32_t)slots;
     float e[3];
@@ -996,7 +1028,10 @@ MP_DEV bool forest_flush(const DevObject* __restrict__ objs, const CanonBuffers&
     return true;
 }
 
-template <int NT, int PT>
+// SO: a frame of one object (and view): its tables are read from the kernel
+// arguments' arrays at index 0 (wave-uniform: scalar registers, no per-step
+// loads) instead of the LDS copies indexed per range.
+template <int NT, int PT, bool SO>
 __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict__ objs,
                                                      const DevView* __restrict__ views, DevParams prm,
                                                      SplitQueues q, CanonBuffers cb, FrameCounters* fc,
@@ -1103,7 +1138,14 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
             int iu = j / 3, iv = j - 3 * (j / 3);
             if (two) spec_sample_sel(pr, j, c, iu, iv);
             float v[5] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
-            if ((int)(c.obj_depth >> 24) < dlim) forest_sample<PT>(s_obj, s_view, s_cps, slots, CS, c, iu, iv, v);
+            if ((int)(c.obj_depth >> 24) < dlim) {
+                if (SO) {
+                    forest_sample<PT>(objs[0], views[0], s_cps, slots, CS, c, iu, iv, v);
+                } else {
+                    const ForestObj& ob = s_obj[c.obj_depth & 0xffffffu];
+                    forest_sample<PT>(ob, s_view[ob.view], s_cps, slots, CS, c, iu, iv, v);
+                }
+            }
 #pragma unroll
             for (int k = 0; k < 5; ++k) s_xch[t * 5 + k] = v[k];
         }
@@ -1125,14 +1167,20 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 const int j = two ? spec_index(d, k / 3, k % 3) : k;
-                const int o = live ? (g * per + j) * 5 : 0;
+                // (issued before the candidate's record is known)
+                const int o = (min(g, n - 1) * per + j) * 5;
                 ex[k] = s_xch[o]; ey[k] = s_xch[o + 1]; ez[k] = s_xch[o + 2];
                 sx[k] = s_xch[o + 3]; sy[k] = s_xch[o + 4];
             }
             if (live) {
-                const ForestObj& ob = s_obj[rec.obj_depth & 0xffffffu];
-                f = bound_decide(ex, ey, ez, sx, sy, rec.umin, rec.vmin, rec.umax, rec.vmax, s_view[ob.view], prm,
-                                 ob.split_limit);
+                if (SO) {
+                    f = bound_decide(ex, ey, ez, sx, sy, rec.umin, rec.vmin, rec.umax, rec.vmax, views[0], prm,
+                                     objs[0].split_limit);
+                } else {
+                    const ForestObj& ob = s_obj[rec.obj_depth & 0xffffffu];
+                    f = bound_decide(ex, ey, ez, sx, sy, rec.umin, rec.vmin, rec.umax, rec.vmax, s_view[ob.view],
+                                     prm, ob.split_limit);
+                }
             }
         }
         int flags = f;
@@ -1189,8 +1237,8 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
             if (kfill + nd > kForestKeys) keys_ok = false;
             if ((flags & 1) && keys_ok) {
                 const int at = kfill + dof + (int)lanes_below(db);
-                const ForestObj& ob = s_obj[rec.obj_depth & 0xffffffu];
-                const uint32_t slot = (ob.gpid_base + (uint32_t)rec.pid - (uint32_t)wid) / (uint32_t)gridDim.x;
+                const uint32_t gb = SO ? objs[0].gpid_base : s_obj[rec.obj_depth & 0xffffffu].gpid_base;
+                const uint32_t slot = (gb + (uint32_t)rec.pid - (uint32_t)wid) / (uint32_t)gridDim.x;
                 s_dk[at] = ((unsigned long long)slot << 48) | canonical_key(rec);
                 cb.forest_out[(size_t)wid * kForestKeys + at] = rec;
             }
@@ -1327,8 +1375,8 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
     uint32_t nb = nbounds;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) nb += (uint32_t)__shfl_xor((int)nb, o);
-    if (lane == 0) atomicAdd(&fc->bounds, (unsigned long long)nb);
-    if (tid == 0) atomicMax(&fc->max_stack, (uint32_t)maxsh);
+    if (lane == 0) atomicAdd(stripe(fc, (uint32_t)(wid * NW + wave), kStripeBounds), (unsigned long long)nb);
+    if (tid == 0) atomicMax(stripe(fc, (uint32_t)wid, kStripeMaxStack), (unsigned long long)maxsh);
 }
 
 size_t split_forest_lds(int stack_cap)
@@ -1339,7 +1387,7 @@ size_t split_forest_lds(int stack_cap)
 int split_forest_occupancy(size_t lds_bytes)
 {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_split_forest<kForestThreads, 2>, kForestThreads,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_split_forest<kForestThreads, 2, false>, kForestThreads,
                                                      lds_bytes) != hipSuccess)
         blocks = 1;
     return blocks < 1 ? 1 : blocks;
@@ -1361,15 +1409,18 @@ hipError_t launch_split_forest(const DevObject* objs, const DevView* views, cons
 {
     if (!split_forest_fits(n_objs, n_views)) return hipErrorInvalidValue;
     const size_t lds = split_forest_lds(q.stack_cap);
-    if (patch_types == 1)
-        hipLaunchKernelGGL((k_split_forest<kForestThreads, 0>), dim3(q.n_groups), dim3(kForestThreads), lds, s,
-                           objs, views, prm, q, cb, fc, n_objs, n_views, ranked);
-    else if (patch_types == 2)
-        hipLaunchKernelGGL((k_split_forest<kForestThreads, 1>), dim3(q.n_groups), dim3(kForestThreads), lds, s,
-                           objs, views, prm, q, cb, fc, n_objs, n_views, ranked);
-    else
-        hipLaunchKernelGGL((k_split_forest<kForestThreads, 2>), dim3(q.n_groups), dim3(kForestThreads), lds, s,
-                           objs, views, prm, q, cb, fc, n_objs, n_views, ranked);
+#define MP_FOREST(PT, SO)                                                                                     \
+    hipLaunchKernelGGL((k_split_forest<kForestThreads, PT, SO>), dim3(q.n_groups), dim3(kForestThreads), lds, s, \
+                       objs, views, prm, q, cb, fc, n_objs, n_views, ranked)
+    const bool one = n_objs == 1;
+    if (patch_types == 1) {
+        if (one) MP_FOREST(0, true); else MP_FOREST(0, false);
+    } else if (patch_types == 2) {
+        if (one) MP_FOREST(1, true); else MP_FOREST(1, false);
+    } else {
+        MP_FOREST(2, false);
+    }
+#undef MP_FOREST
     return hipGetLastError();
 }
 
@@ -1407,7 +1458,7 @@ __global__ __launch_bounds__(SEG) void k_bound_segment(const DevObject* __restri
         blk_counts[2 * blockIdx.x] = a;
         blk_counts[2 * blockIdx.x + 1] = b;
         uint32_t m = n - blockIdx.x * SEG;
-        atomicAdd(&fc->bounds, (unsigned long long)(m < SEG ? m : SEG));
+        atomicAdd(stripe(fc, blockIdx.x, kStripeBounds), (unsigned long long)(m < SEG ? m : SEG));
     }
 }
 
@@ -2566,7 +2617,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (slow) atomicAdd(&fc->slow_blocks, slow);
     if (uncached) atomicAdd(&fc->uncached_octaves, uncached);
     natom = (uint32_t)wave_sum_i((int)natom);
-    if (lane == 0 && natom) atomicAdd(&fc->atomics, (unsigned long long)natom);
+    if (lane == 0 && natom) atomicAdd(stripe(fc, blockIdx.x, kStripeAtomics), (unsigned long long)natom);
 }
 
 template <int N, int MODE, int NT>
@@ -2745,7 +2796,10 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
                                                  FrameEnd fe)
 {
     __shared__ float s_tab[256];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) end_frame(fe);   // the resolve reads no counter
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 64 && fe.fc) {   // the resolve reads no counter
+        fold_stripes(fe.fc);
+        if (threadIdx.x == 0) end_frame(fe);
+    }
     if (blockIdx.y == 0) {
         // per-patch counts of a ranked frame (nothing reads them after the order)
         const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -3328,7 +3382,8 @@ __global__ __launch_bounds__(256) void k_level(const DevObject* __restrict__ obj
         }
         nb += (uint32_t)__popcll(__ballot(valid));
     }
-    if (lane == 0 && nb) atomicAdd(&fc->bounds, (unsigned long long)nb);
+    if (lane == 0 && nb) atomicAdd(stripe(fc, blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6), kStripeBounds),
+                                   (unsigned long long)nb);
 }
 
 // Same level step with nine lanes per range: lane 9g+s of a wave evaluates
@@ -3426,7 +3481,8 @@ __global__ __launch_bounds__(256) void k_level9(const DevObject* __restrict__ ob
         }
         nb += (uint32_t)__popcll(__ballot(leader));
     }
-    if (lane == 0 && nb) atomicAdd(&fc->bounds, (unsigned long long)nb);
+    if (lane == 0 && nb) atomicAdd(stripe(fc, blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6), kStripeBounds),
+                                   (unsigned long long)nb);
 }
 
 hipError_t launch_split_rebalanced(const DevObject* objs, const DevView* views, const DevParams& prm,

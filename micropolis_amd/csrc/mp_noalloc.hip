@@ -140,7 +140,7 @@ extern "C" int mp_render_noalloc(const mp_config* cfg, const float* xyz, size_t 
 
     Carve c;
     const size_t o_obj = c.take(sizeof(DevObject)), o_view = c.take(sizeof(DevView));
-    const size_t o_fc = c.take(sizeof(FrameCounters)), o_share = c.take(sizeof(ShareState));
+    const size_t o_fc = c.take(kFrameCountersBytes), o_share = c.take(sizeof(ShareState));
     const size_t o_tab = c.take(256 * sizeof(float));
     const size_t o_cps = c.take(n_points * sizeof(float4));
     const size_t o_queue = c.take(std::max<size_t>(1, stride * (size_t)groups) * sizeof(RangeRec));

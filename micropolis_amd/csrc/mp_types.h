@@ -94,4 +94,16 @@ struct FrameCounters {
     uint32_t uncached_octaves;   // noise octaves evaluated by hashing
 };
 
+// The counters every work-group of a large launch adds to (coverage samples,
+// bound() evaluations, the stack maximum) are accumulated in kStripes
+// stripes, one per 128-byte line, that follow the FrameCounters in the same
+// allocation: atomics from thousands of work-groups to one address are
+// performed one after another (4096 of them at a launch's end cost ~50 us),
+// to 64 addresses in parallel.  The frame end folds the stripes into the
+// counters.
+constexpr int kStripes = 64;
+constexpr int kStripeWords = 16;
+constexpr size_t kFrameCountersBytes = sizeof(FrameCounters) + (size_t)kStripes * kStripeWords * 8;
+enum { kStripeAtomics = 0, kStripeBounds = 1, kStripeMaxStack = 2, kStripeUsed = 3 };
+
 }  // namespace mp
