@@ -265,7 +265,9 @@ typedef struct mp_tuning {
     int32_t raster_per_cu;   /* raster work-groups per CU (0 = occupancy limit) */
     int32_t raster_chunk;    /* raster range schedule: 0 static stride (no
                                 atomics), -1 dynamic chunks sized from the
-                                range count, > 0 dynamic chunks of that size */
+                                range count, > 0 dynamic chunks of that size,
+                                <= -2 static chunks of -raster_chunk
+                                consecutive ranges (no atomics) */
     int32_t split_levels;    /* split_engine 1: level launches before the
                                 persistent tail (0 = as many as the frame needs) */
     int32_t split_budget;    /* LOCAL, split_engine 0, no work sharing: steps

@@ -1906,7 +1906,7 @@ extern "C" int mp_set_tuning(mp_context* ctx, const mp_tuning* t)
 {
     if (!ctx || !t) return MP_E_INVALID;
     if (t->split_engine < 0 || t->split_engine > 2 || t->raster_per_cu < 0 || t->raster_per_cu > 64 ||
-        t->raster_chunk < -1 || t->raster_chunk > 4096 || t->split_levels < 0 || t->split_levels > 23 ||
+        t->raster_chunk < -4096 || t->raster_chunk > 4096 || t->split_levels < 0 || t->split_levels > 23 ||
         t->split_budget < 0 || t->split_budget > 4096 ||
         (t->split_waves != 0 && t->split_waves != 2 && t->split_waves != 3))
         return fail(ctx, MP_E_INVALID, "tuning out of range");
@@ -1945,7 +1945,7 @@ extern "C" int mp_autotune(mp_context* ctx, int32_t replays, mp_tuning* chosen)
     const int occ = raster_grid(ctx->cfg.reyes_patch_size, ctx->num_cus, ctx->prm) / ctx->num_cus;
     std::vector<mp_tuning> cand;
     for (int per_cu : { occ, std::max(1, occ / 2) }) {
-        for (int chunk : { 0, -1 }) {
+        for (int chunk : { 0, -1, -16 }) {   // stride, dynamic, static chunks
             mp_tuning c = cur;
             c.raster_per_cu = per_cu;
             c.raster_chunk = chunk;

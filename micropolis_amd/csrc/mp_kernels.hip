@@ -2441,19 +2441,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
     // Range schedule.  Static (prm.raster_chunk == 0): work-group g takes
     // ranges g, g + grid, ... with no atomics, the next range known ahead.
-    // Dynamic: chunks claimed from one counter, one atomic per chunk (a
-    // returning atomic on one address serialises across the device, so the
-    // chunk must stay large against the range count); raster_chunk < 0
-    // sizes it from the range count, > 0 fixes it.
-    const bool dyn = prm.raster_chunk != 0 && count == 0;
-    const uint32_t chunk = prm.raster_chunk > 0 ? (uint32_t)prm.raster_chunk
+    // Static chunks (raster_chunk <= -2): work-group g takes the chunks of
+    // -raster_chunk consecutive ranges g, g + grid, ... (neighbouring ranges
+    // share patches and objects in the caches), no atomics.  Dynamic: chunks
+    // claimed from one counter, one atomic per chunk (a returning atomic on
+    // one address serialises across the device, so the chunk must stay large
+    // against the range count); raster_chunk == -1 sizes it from the range
+    // count, > 0 fixes it.
+    const bool blk = prm.raster_chunk <= -2 && count == 0;
+    const bool dyn = prm.raster_chunk != 0 && !blk && count == 0;
+    const uint32_t chunk = blk ? (uint32_t)(-prm.raster_chunk)
+                         : prm.raster_chunk > 0 ? (uint32_t)prm.raster_chunk
                                                 : max(8u, min(64u, total / (gridDim.x * 4u)));
-    uint32_t r, r_end;
+    uint32_t r, r_end, cstart = blockIdx.x * chunk;
     if (dyn) {
         if (tid == 0) s_next = atomicAdd(&fc->raster_next, chunk);
         __syncthreads();
         r = s_next;
         r_end = min(r + chunk, total);
+    } else if (blk) {
+        r = cstart;
+        r_end = min(cstart + chunk, total);
     } else {
         r = first + blockIdx.x;
         r_end = total;
@@ -2473,12 +2481,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // next range (the following chunk is claimed one range ahead) and its
         // control points: loaded now, stored to the other LDS buffer after
         // this range, so the load latency hides behind this range's work
-        uint32_t rn = dyn ? r + 1 : r + gridDim.x, rn_end = r_end;
+        uint32_t rn = (dyn || blk) ? r + 1 : r + gridDim.x, rn_end = r_end;
         if (dyn && rn >= r_end) {
             if (tid == 0) s_next = atomicAdd(&fc->raster_next, chunk);
             __syncthreads();
             rn = s_next;
             rn_end = min(rn + chunk, total);
+        } else if (blk && rn >= r_end) {
+            cstart += gridDim.x * chunk;
+            rn = cstart;
+            rn_end = min(cstart + chunk, total);
         }
         bool have_pf = false;
         if (rn < total) {

@@ -76,8 +76,9 @@ struct DevParams {
     int32_t noise_cache;         // cache lattice values per range in LDS
     int32_t noise_wave_cells;    // hash wave-uniform lattice cells once per wave
     int32_t raster_one_wave;     // N = 16: one-wave raster work-groups
-    int32_t raster_chunk;        // 0: static range stride; < 0: dynamic, sized
-                                 // from the range count; > 0: dynamic, fixed
+    int32_t raster_chunk;        // 0: static range stride; -1: dynamic, sized
+                                 // from the range count; > 0: dynamic, fixed;
+                                 // <= -2: static chunks of -raster_chunk ranges
 };
 
 // Device-side frame counters, copied back once per frame.

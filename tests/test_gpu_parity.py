@@ -89,7 +89,7 @@ def test_every_scheduling_gives_the_same_frame(oracle):
     r, got = render_plain(w)
     assert_frame_equal(got, ref, "c4-small/default")
     for engine, sharing, chunk, levels in ((0, 0, 0, 0), (0, 1, -1, 0), (1, 0, 0, 0), (1, 1, -1, 0),
-                                           (0, 0, 7, 0), (1, 0, 0, 2), (1, 1, 0, 3), (2, 0, 0, 0), (2, 0, -1, 0)):
+                                           (0, 0, 7, 0), (1, 0, 0, 2), (1, 1, 0, 3), (2, 0, 0, 0), (2, 0, -1, 0), (2, 0, -16, 0)):
         r.set_tuning(split_engine=engine, work_sharing=sharing, raster_chunk=chunk, split_levels=levels)
         r.replay(2, use_graph=True)
         assert_frame_equal(_frame_of(r), ref, "c4-small/engine%d-sharing%d-chunk%d-levels%d"
