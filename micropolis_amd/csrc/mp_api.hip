@@ -308,6 +308,7 @@ void mp::make_view(const mp_projection& pr, const float clear[4], int subpixel_b
     v.clear_linear[3] = 1000.0f;
     v.clear_rgba8 = display_byte_host(v.clear_linear[0]) | (display_byte_host(v.clear_linear[1]) << 8) |
                     (display_byte_host(v.clear_linear[2]) << 16) | (255u << 24);
+    v.solo_obj = -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -799,11 +800,13 @@ static int upload_tables(mp_context* ctx)
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return ctx->objects[a].view < ctx->objects[b].view; });
     std::vector<uint32_t> first(nv, 0), end(nv, 0);
+    std::vector<int> solo(nv, -1);
     uint32_t g = 0;
     ctx->patch_types = 0;
     int oi = 0;
     for (int v = 0; v < nv; ++v) {
         first[v] = g;
+        const int oi0 = oi;
         while (oi < no && ctx->objects[order[oi]].view == v) {
             const ObjectReq& o = ctx->objects[order[oi]];
             const PatchSet& ps = ctx->patches[o.handle];
@@ -822,6 +825,7 @@ static int upload_tables(mp_context* ctx)
             g += (uint32_t)ps.n_patches;
             ++oi;
         }
+        if (oi == oi0 + 1) solo[v] = oi0;
         end[v] = g;
     }
     ctx->n_gpids = g;
@@ -849,6 +853,7 @@ static int upload_tables(mp_context* ctx)
         dv.first_gpid = first[v];
         dv.gpid_end = end[v];
         dv.pixel_offset = pix;
+        dv.solo_obj = solo[v];
         pix += (uint32_t)dv.width * (uint32_t)dv.height;
     }
     if (nv) HIPCHK(ctx, hipMemcpyAsync(ctx->d_views, ctx->h_views, nv * sizeof(DevView), hipMemcpyHostToDevice, ctx->stream));

@@ -183,6 +183,28 @@ MP_DEV void eval_bezier(const float4* __restrict__ cp, float u, float v, float p
     eval_bezier_w(cp, a, b, p);
 }
 
+// Two points of one bicubic Bezier patch: each control point is read once
+// for both sums, and each sum keeps eval_bezier's order of operations.
+MP_DEV void eval_bezier2(const float4* __restrict__ cp, float u0, float v0, float u1, float v1,
+                         float p0[4], float p1[4])
+{
+    float a0[4], b0[4], a1[4], b1[4];
+    bern_u(u0, 1.0f - u0, a0);
+    bern_v(v0, 1.0f - v0, b0);
+    bern_u(u1, 1.0f - u1, a1);
+    bern_v(v1, 1.0f - v1, b1);
+    p0[0] = p0[1] = p0[2] = p0[3] = 0.0f;
+    p1[0] = p1[1] = p1[2] = p1[3] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float4 c = cp[4 * i + j];
+            acc4(p0, a0[i] * b0[j], c);
+            acc4(p1, a1[i] * b1[j], c);
+        }
+}
+
 MP_DEV float nz(float x) { return (x != x) ? 0.0f : x; }
 
 MP_DEV float4 greg_f(float nx, float ny, float nz_, float nw, float d)

@@ -2369,13 +2369,54 @@ MP_DEV void dice_grid(const float4* __restrict__ s_cp, float* __restrict__ s_col
         }
         __syncthreads();
     }
+    // eye position, sub-pixel coordinate and depth of grid point g from its
+    // patch point P; stored when the point exists (store)
+    auto project = [&](int g, const float P[4], bool store) {
+        float e[4], q[4];
+        mat4_mul(ob.mv, P, e);
+        perspective_mul(vw.proj, e, q);
+        // (x * W) / 2 + W / 2 (dice.cl:71-72): halving is exact, and a product
+        // too small for it to be is absorbed by the addend (>= 32), so the
+        // fused form equals the written one
+        const int32_t px = f2i_sat(mad((q[0] / q[3]) * (float)vw.vs_x, 0.5f, (float)(vw.vs_x / 2)));
+        const int32_t py = f2i_sat(mad((q[1] / q[3]) * (float)vw.vs_y, 0.5f, (float)(vw.vs_y / 2)));
+        if (store) {
+            set_pos(s_pos[g], e[0], e[1], e[2], e[3]);
+            s_pxl[g] = make_int2(px, py);
+            s_dep[g] = q[2] / q[3];
+        }
+    };
+    if (MODE == 0 && ob.type == MP_TYPE_BEZIER) {
+        // undisplaced Bezier: each lane takes points gb + lane and, while the
+        // grid has them, gb + NT + lane in one pass, every control point read
+        // from LDS once for both
+        for (int gb = wave * 64; gb < G; gb += 2 * NT) {
+            const int g0 = min(gb + lane, G - 1);
+            const float u0 = mad(du, (float)(g0 % G1) / (float)N, rec.umin);
+            const float v0 = mad(dv, (float)(g0 / G1) / (float)N, rec.vmin);
+            float P0[4];
+            if (gb + NT < G) {
+                const int g1 = min(gb + NT + lane, G - 1);
+                const float u1 = mad(du, (float)(g1 % G1) / (float)N, rec.umin);
+                const float v1 = mad(dv, (float)(g1 / G1) / (float)N, rec.vmin);
+                float P1[4];
+                eval_bezier2(s_cp, u0, v0, u1, v1, P0, P1);
+                project(g0, P0, gb + lane < G);
+                project(g1, P1, gb + NT + lane < G);
+            } else {
+                eval_bezier(s_cp, u0, v0, P0);
+                project(g0, P0, gb + lane < G);
+            }
+        }
+        return;
+    }
     for (int gb = wave * 64; gb < G; gb += NT) {
         // control points re-read from LDS each pass (registers for the kept
         // lattice windows)
         asm volatile("" ::: "memory");
         const int g = min(gb + lane, G - 1);
         const int nu = g % G1, nv = g / G1;
-        float P[4], e[4], q[4];
+        float P[4];
         if (disp_cols) {
             const float4 A = s_wt[nu], Bv = s_wt[G1 + 2 * nv], Dv = s_wt[G1 + 2 * nv + 1];
             const float a[4] = { A.x, A.y, A.z, A.w }, b[4] = { Bv.x, Bv.y, Bv.z, Bv.w };
@@ -2387,18 +2428,7 @@ MP_DEV void dice_grid(const float4* __restrict__ s_cp, float* __restrict__ s_col
             if (MODE == 1) displaced_point(s_cp, ob.type, uu, vv, prm, ob.disp_amplitude, P);
             else eval_patch(s_cp, ob.type, uu, vv, P);
         }
-        mat4_mul(ob.mv, P, e);
-        perspective_mul(vw.proj, e, q);
-        // (x * W) / 2 + W / 2 (dice.cl:71-72): halving is exact, and a product
-        // too small for it to be is absorbed by the addend (>= 32), so the
-        // fused form equals the written one
-        const int32_t px = f2i_sat(mad((q[0] / q[3]) * (float)vw.vs_x, 0.5f, (float)(vw.vs_x / 2)));
-        const int32_t py = f2i_sat(mad((q[1] / q[3]) * (float)vw.vs_y, 0.5f, (float)(vw.vs_y / 2)));
-        if (gb + lane < G) {
-            set_pos(s_pos[g], e[0], e[1], e[2], e[3]);
-            s_pxl[g] = make_int2(px, py);
-            s_dep[g] = q[2] / q[3];
-        }
+        project(g, P, gb + lane < G);
     }
 }
 
@@ -2800,6 +2830,10 @@ MP_DEV uint32_t display_byte(float c, const float* tab)
     return (uint32_t)k;
 }
 
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ views,
                                                  const RangeRec* __restrict__ canon,
                                                  const DevObject* __restrict__ objs,
@@ -2820,38 +2854,85 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
     if (threadIdx.x < 255) s_tab[threadIdx.x] = display_tab[threadIdx.x];
     __syncthreads();
     const DevView& vw = views[blockIdx.y];
-    const uint32_t n = (uint32_t)vw.grid_w * (uint32_t)vw.grid_h * 64u;
+    // four consecutive keys per thread (half a tile row): the key loads and
+    // then the gathers of the four pixels are in flight together
+    const uint32_t n4 = (uint32_t)vw.grid_w * (uint32_t)vw.grid_h * 16u;
     const float ac[4] = { 0.015f, 0.015f, 0.015f, 1.0f };
     const uint32_t qshift = (uint32_t)__builtin_ctz((unsigned)Q);   // Q = N*N is a power of two
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const unsigned long long key = gptr(vw.keys)[i];
-        gptr(vw.keys)[i] = MP_EMPTY_KEY;
+    const size_t qbase = (size_t)vw.range_base * Q;
+    // a view of one object takes its colour without the range -> object gather
+    const int solo = vw.solo_obj;
+    // whole-row stores of the four pixels need 16-byte aligned rows
+    const bool vec = ((reinterpret_cast<uintptr_t>(vw.rgba8) | reinterpret_cast<uintptr_t>(vw.depth) |
+                       reinterpret_cast<uintptr_t>(vw.ids)) & 15u) == 0 && (vw.width & 3) == 0;
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n4; j += gridDim.x * 256) {
+        const uint32_t i = j * 4u;
+        MP_GLOBAL u64x2* kp = gptr(reinterpret_cast<u64x2*>(vw.keys + i));
+        const u64x2 ka = kp[0], kb = kp[1];
+        const u64x2 empty = { MP_EMPTY_KEY, MP_EMPTY_KEY };
+        kp[0] = empty;
+        kp[1] = empty;
+        const unsigned long long key[4] = { ka.x, ka.y, kb.x, kb.y };
         const uint32_t t = i >> 6, l = i & 63;
         const int X = (int)(t % (uint32_t)vw.grid_w) * 8 + (int)(l & 7);
         const int Y = (int)(t / (uint32_t)vw.grid_w) * 8 + (int)(l >> 3);
         if (X >= vw.width || Y >= vw.height) continue;
-        const size_t o = (size_t)Y * vw.width + X;
-        float c[4];
-        float d = 1.0f;
-        uint32_t id = 0xFFFFFFFFu, bytes = vw.clear_rgba8;
-        if (key == MP_EMPTY_KEY) {
-            c[0] = vw.clear_linear[0]; c[1] = vw.clear_linear[1];
-            c[2] = vw.clear_linear[2]; c[3] = vw.clear_linear[3];
-        } else {
-            id = (uint32_t)key;
-            d = unmono_bits((uint32_t)(key >> 32));
-            const uint32_t rr = vw.range_base + (id >> qshift);
-            const float s = quad_store[(size_t)vw.range_base * Q + id];
-            const DevObject& ob = objs[canon[rr].obj_depth & 0xffffffu];
+        float s[4];
+        int obj[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) c[k] = mad(s, ob.color[k], ac[k] * ob.color[k]) + 0.0f;
-            bytes = display_byte(c[0], s_tab) | (display_byte(c[1], s_tab) << 8) |
-                    (display_byte(c[2], s_tab) << 16) | (255u << 24);
+        for (int k = 0; k < 4; ++k) {
+            s[k] = 0.0f;
+            obj[k] = solo;
+            if (key[k] != MP_EMPTY_KEY) {
+                const uint32_t id = (uint32_t)key[k];
+                s[k] = quad_store[qbase + id];
+                if (solo < 0) obj[k] = (int)(canon[vw.range_base + (id >> qshift)].obj_depth & 0xffffffu);
+            }
         }
-        *gptr(reinterpret_cast<uint32_t*>(vw.rgba8) + o) = bytes;
-        gptr(vw.depth)[o] = d;
-        gptr(vw.ids)[o] = id;
-        if (write_rgba32) stg4(reinterpret_cast<float4*>(vw.rgba32) + o, make_float4(c[0], c[1], c[2], c[3]));
+        uint32_t bytes[4], ids[4];
+        float dep[4], c[4][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            dep[k] = 1.0f;
+            ids[k] = 0xFFFFFFFFu;
+            bytes[k] = vw.clear_rgba8;
+            if (key[k] == MP_EMPTY_KEY) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) c[k][m] = vw.clear_linear[m];
+            } else {
+                ids[k] = (uint32_t)key[k];
+                dep[k] = unmono_bits((uint32_t)(key[k] >> 32));
+                const DevObject& ob = objs[obj[k]];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) c[k][m] = mad(s[k], ob.color[m], ac[m] * ob.color[m]) + 0.0f;
+                bytes[k] = display_byte(c[k][0], s_tab) | (display_byte(c[k][1], s_tab) << 8) |
+                           (display_byte(c[k][2], s_tab) << 16) | (255u << 24);
+            }
+        }
+        const size_t o = (size_t)Y * vw.width + X;
+        if (vec) {
+            // width % 4 == 0 and X % 4 == 0: all four pixels are in the row
+            const u32x4 b4 = { bytes[0], bytes[1], bytes[2], bytes[3] };
+            const f32x4 d4 = { dep[0], dep[1], dep[2], dep[3] };
+            const u32x4 i4 = { ids[0], ids[1], ids[2], ids[3] };
+            *gptr(reinterpret_cast<u32x4*>(vw.rgba8 + 4 * o)) = b4;
+            *gptr(reinterpret_cast<f32x4*>(vw.depth + o)) = d4;
+            *gptr(reinterpret_cast<u32x4*>(vw.ids + o)) = i4;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (X + k >= vw.width) break;
+                *gptr(reinterpret_cast<uint32_t*>(vw.rgba8) + o + k) = bytes[k];
+                gptr(vw.depth)[o + k] = dep[k];
+                gptr(vw.ids)[o + k] = ids[k];
+            }
+        }
+        if (write_rgba32) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (X + k < vw.width)
+                    stg4(reinterpret_cast<float4*>(vw.rgba32) + o + k, make_float4(c[k][0], c[k][1], c[k][2], c[k][3]));
+        }
     }
 }
 
@@ -2860,9 +2941,9 @@ hipError_t launch_resolve(const DevView* views, int n_views, int max_pixels,
                           int patch_size, int write_rgba32, const float* display_tab, hipStream_t s,
                           const FrameEnd& fe)
 {
-    // one key per thread: every pixel's gather chain (range, object,
+    // four keys per thread: every pixel's gather chain (range, object,
     // Lambert term) is in flight at once
-    int bx = (max_pixels + 255) / 256;
+    int bx = ((max_pixels + 3) / 4 + 255) / 256;
     if (bx > 65535) bx = 65535;
     if (bx < 1) bx = 1;
     FrameEnd fr = fe;

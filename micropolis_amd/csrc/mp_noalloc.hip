@@ -207,6 +207,7 @@ extern "C" int mp_render_noalloc(const mp_config* cfg, const float* xyz, size_t 
     vw.first_gpid = 0;
     vw.gpid_end = P;
     vw.pixel_offset = 0;
+    vw.solo_obj = 0;
     memcpy(t.tab, s_tab, sizeof(t.tab));
     const DevParams prm = device_params(cfg);
 

@@ -45,6 +45,7 @@ struct DevView {
     uint32_t gpid_end;           // one past the global patch index of its last object
     uint32_t pixel_offset;       // first pixel of this view in multi-view buffers
     uint32_t clear_rgba8;        // display bytes of the clear colour (alpha 255)
+    int32_t solo_obj;            // the view's only object, -1 when it has several or none
 };
 
 // One draw_patches call: patch buffer + modelview + colour.
