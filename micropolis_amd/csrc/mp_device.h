@@ -183,9 +183,10 @@ MP_DEV void eval_bezier(const float4* __restrict__ cp, float u, float v, float p
     eval_bezier_w(cp, a, b, p);
 }
 
-// Two points of one bicubic Bezier patch: each control point is read once
-// for both sums, and each sum keeps eval_bezier's order of operations.
-MP_DEV void eval_bezier2(const float4* __restrict__ cp, float u0, float v0, float u1, float v1,
+// Two bicubic Bezier points, p0 of the patch cp0 and p1 of cp1, each sum in
+// eval_bezier's order of operations; with cp0 == cp1 each control point is
+// read once for both.
+MP_DEV void eval_bezier2(const float4* cp0, const float4* cp1, float u0, float v0, float u1, float v1,
                          float p0[4], float p1[4])
 {
     float a0[4], b0[4], a1[4], b1[4];
@@ -199,9 +200,8 @@ MP_DEV void eval_bezier2(const float4* __restrict__ cp, float u0, float v0, floa
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float4 c = cp[4 * i + j];
-            acc4(p0, a0[i] * b0[j], c);
-            acc4(p1, a1[i] * b1[j], c);
+            acc4(p0, a0[i] * b0[j], cp0[4 * i + j]);
+            acc4(p1, a1[i] * b1[j], cp1[4 * i + j]);
         }
 }
 

@@ -2400,7 +2400,7 @@ MP_DEV void dice_grid(const float4* __restrict__ s_cp, float* __restrict__ s_col
                 const float u1 = mad(du, (float)(g1 % G1) / (float)N, rec.umin);
                 const float v1 = mad(dv, (float)(g1 / G1) / (float)N, rec.vmin);
                 float P1[4];
-                eval_bezier2(s_cp, u0, v0, u1, v1, P0, P1);
+                eval_bezier2(s_cp, s_cp, u0, v0, u1, v1, P0, P1);
                 project(g0, P0, gb + lane < G);
                 project(g1, P1, gb + NT + lane < G);
             } else {
@@ -2432,6 +2432,63 @@ MP_DEV void dice_grid(const float4* __restrict__ s_cp, float* __restrict__ s_col
     }
 }
 
+// Undisplaced Bezier dice (dice.cl:38-79) of range a, or of ranges a and b
+// of one object (two), by one wave into the flat grid arrays: a's points at
+// [0, G), b's at [G, 2G).  A grid of 64 < G <= 96 points takes two lane
+// passes per range alone; two ranges take three: a's and b's first 64
+// points side by side, then the G - 64 left of each, a's on lanes 0..31 and
+// b's on lanes 32..63.
+template <int N, typename PT>
+MP_DEV void dice_bezier_pair(const float4* cpa, const float4* cpb, const RangeRec& ra, const RangeRec& rb,
+                             bool two, const DevObject& ob, const DevView& vw,
+                             PT* __restrict__ s_pos, int2* __restrict__ s_pxl, float* __restrict__ s_dep)
+{
+    constexpr int G1 = N + 1;
+    constexpr int G = G1 * G1;
+    static_assert(G > 64 && 2 * (G - 64) <= 64, "grid of 64 < G <= 96 points");
+    const int lane = threadIdx.x & 63;
+    auto project = [&](int f, const float P[4], bool store) {
+        float e[4], q[4];
+        mat4_mul(ob.mv, P, e);
+        perspective_mul(vw.proj, e, q);
+        // the fused form of (x * W) / 2 + W / 2 (dice.cl:71-72), as in dice_grid
+        const int32_t px = f2i_sat(mad((q[0] / q[3]) * (float)vw.vs_x, 0.5f, (float)(vw.vs_x / 2)));
+        const int32_t py = f2i_sat(mad((q[1] / q[3]) * (float)vw.vs_y, 0.5f, (float)(vw.vs_y / 2)));
+        if (store) {
+            set_pos(s_pos[f], e[0], e[1], e[2], e[3]);
+            s_pxl[f] = make_int2(px, py);
+            s_dep[f] = q[2] / q[3];
+        }
+    };
+    // umin + (umax - umin) * i / N, fused as in dice_grid
+    const float dua = ra.umax - ra.umin, dva = ra.vmax - ra.vmin;
+    const float dub = rb.umax - rb.umin, dvb = rb.vmax - rb.vmin;
+    float P0[4], P1[4];
+    if (!two) {
+        const int g1 = min(64 + lane, G - 1);
+        eval_bezier2(cpa, cpa, mad(dua, (float)(lane % G1) / (float)N, ra.umin),
+                     mad(dva, (float)(lane / G1) / (float)N, ra.vmin),
+                     mad(dua, (float)(g1 % G1) / (float)N, ra.umin),
+                     mad(dva, (float)(g1 / G1) / (float)N, ra.vmin), P0, P1);
+        project(lane, P0, true);
+        project(g1, P1, 64 + lane < G);
+        return;
+    }
+    const float tu = (float)(lane % G1) / (float)N, tv = (float)(lane / G1) / (float)N;
+    eval_bezier2(cpa, cpb, mad(dua, tu, ra.umin), mad(dva, tv, ra.vmin), mad(dub, tu, rb.umin),
+                 mad(dvb, tv, rb.vmin), P0, P1);
+    project(lane, P0, true);
+    project(G + lane, P1, true);
+    const bool hb = lane >= 32;
+    const int k = lane & 31;
+    const int g = 64 + min(k, G - 65);
+    const float du = hb ? dub : dua, dv = hb ? dvb : dva;
+    const float umin = hb ? rb.umin : ra.umin, vmin = hb ? rb.vmin : ra.vmin;
+    eval_bezier(hb ? cpb : cpa, mad(du, (float)(g % G1) / (float)N, umin), mad(dv, (float)(g / G1) / (float)N, vmin),
+                P0);
+    project((hb ? G : 0) + g, P0, k < G - 64);
+}
+
 // NT threads per work-group: N*N (one wave per 8x8 block of quads) or 64
 // (one wave dices the whole grid and shades the blocks in turn: no waiting
 // between waves at the range's barriers).
@@ -2447,13 +2504,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     constexpr int G = G1 * G1;
     constexpr int Q = N * N;
     constexpr int B = N / 8;
-    __shared__ float4 s_cpb[2][20];   // control points: current range / next range
+    // One-wave work-groups of 9 x 9 undisplaced grids dice two ranges of one
+    // Bezier object together (three lane passes instead of four).
+    constexpr bool kPair = MODE == 0 && NT == 64 && G > 64 && 2 * (G - 64) <= 64;
+    constexpr int GS = kPair ? 2 * G : G;
+    __shared__ float4 s_cpb[kPair ? 1 : 2][20];   // control points: current range / next range
+    __shared__ float4 s_cq[kPair ? 2 : 1][2][20];  // pairs: [buffer][first, second range]
     // eye-space x, y, z per grid point (the LDS-cached noise path also keeps
     // the object-space w there between its passes)
     typedef typename std::conditional<MODE == 2, float4, EyePos>::type PosT;
-    __shared__ PosT s_pos[G];
-    __shared__ int2 s_pxl[G];
-    __shared__ float s_dep[G];
+    __shared__ PosT s_pos[GS];
+    __shared__ int2 s_pxl[GS];
+    __shared__ float s_dep[GS];
     __shared__ float4 s_nrm[G];
     __shared__ float s_box[NT / 64][6];
     __shared__ int s_geo[kCachedOctaves][6];
@@ -2496,7 +2558,85 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         r = first + blockIdx.x;
         r_end = total;
     }
-    if (r < total) {
+    NoiseWindows nw;   // lattice windows of the low octaves, kept across passes and ranges
+    if (MODE == 1) nw.reset(s_win[wave]);
+    // the range after x (chunk end x_end) in this work-group's schedule
+    auto next = [&](uint32_t& x, uint32_t& x_end) {
+        uint32_t xn = (dyn || blk) ? x + 1 : x + gridDim.x;
+        if (dyn && xn >= x_end) {
+            if (tid == 0) s_next = atomicAdd(&fc->raster_next, chunk);
+            __syncthreads();
+            xn = s_next;
+            x_end = min(xn + chunk, total);
+        } else if (blk && xn >= x_end) {
+            cstart += gridDim.x * chunk;
+            xn = cstart;
+            x_end = min(cstart + chunk, total);
+        }
+        x = xn;
+    };
+    // control points of ranges x (lanes 0..31) and y (lanes 32..63) into pf
+    auto fetch_two = [&](uint32_t x, uint32_t y, float4& pf) {
+        const uint32_t z = lane < 32 ? x : y;
+        if (z < total) {
+            const RangeRec rz = canon[z];
+            const DevObject& oz = objs[rz.obj_depth & 0xffffffu];
+            const int ncp = oz.type == MP_TYPE_GREGORY ? 20 : 16;
+            if ((lane & 31) < ncp) {
+                pf = ldg4(oz.cps + (size_t)rz.pid * ncp + (lane & 31));
+                return true;
+            }
+        }
+        return false;
+    };
+    if constexpr (kPair) {
+        // a: the current range, b: the one after it in the schedule
+        uint32_t a = r, b = r, b_end = r_end;
+        if (a < total) next(b, b_end);
+        else b = total;
+        float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (fetch_two(a, b, pf)) s_cq[0][lane >> 5][lane & 31] = pf;
+        __syncthreads();
+        int qb = 0;
+        while (a < total) {
+            const RangeRec ra = canon[a];
+            const uint32_t oa = ra.obj_depth & 0xffffffu;
+            const DevObject& ob = objs[oa];
+            const DevView& vw = views[ob.view];
+            RangeRec rb = ra;
+            bool two = false;
+            if (b < total) {
+                rb = canon[b];
+                two = (rb.obj_depth & 0xffffffu) == oa && ob.type == MP_TYPE_BEZIER;
+            }
+            // the next window: c after this one or two, d after c; their
+            // control points load now and go to the other buffer after the dice
+            uint32_t c = b, c_end = b_end;
+            if (two && c < total) next(c, c_end);
+            uint32_t d = c, d_end = c_end;
+            if (c < total) next(d, d_end);
+            else d = total;
+            const bool have_pf = fetch_two(c, d, pf);
+            if (ob.type == MP_TYPE_BEZIER)
+                dice_bezier_pair<N>(s_cq[qb][0], s_cq[qb][1], ra, rb, two, ob, vw, s_pos, s_pxl, s_dep);
+            else
+                dice_grid<N, 0, NT>(s_cq[qb][0], s_col, s_wt, ra, ob, vw, prm, s_pos, s_pxl, s_dep, nw);
+            __syncthreads();
+            if (have_pf) s_cq[qb ^ 1][lane >> 5][lane & 31] = pf;
+            // one shading code path for both ranges (instruction cache)
+#pragma nounroll
+            for (int k = 0; k <= (int)two; ++k)
+                shade_sample<N>(s_pos + k * G, s_pxl + k * G, s_dep + k * G, lane & 7, lane >> 3, lane, vw, prm,
+                                k ? b : a, quad_store, slow, natom);
+            a = c;
+            b = d;
+            b_end = d_end;
+            qb ^= 1;
+            __syncthreads();
+        }
+        r = total;   // the loop below has nothing left
+    }
+    if (!kPair && r < total) {
         const RangeRec rec0 = canon[r];
         const DevObject& ob0 = objs[rec0.obj_depth & 0xffffffu];
         const int ncp0 = ob0.type == MP_TYPE_GREGORY ? 20 : 16;
@@ -2505,8 +2645,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __syncthreads();
     int buf = 0;
     float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    NoiseWindows nw;   // lattice windows of the low octaves, kept across passes and ranges
-    if (MODE == 1) nw.reset(s_win[wave]);
     while (r < total) {
         // next range (the following chunk is claimed one range ahead) and its
         // control points: loaded now, stored to the other LDS buffer after
