@@ -2201,12 +2201,11 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
     const int g0 = u + v * G1;
     const int sh = prm.subpixel_bits;
     // ---- shade (reyes.cl:84-199); one wave = one 8x8 block ----
-    const float4 p0 = eye4(s_pos[g0]), p1 = eye4(s_pos[g0 + 1]), p2 = eye4(s_pos[g0 + G1]),
-                 p3 = eye4(s_pos[g0 + G1 + 1]);
+    const float z0 = s_pos[g0].z, z1 = s_pos[g0 + 1].z, z2 = s_pos[g0 + G1].z, z3 = s_pos[g0 + G1 + 1].z;
     const int2 x0 = s_pxl[g0], x1 = s_pxl[g0 + 1], x2 = s_pxl[g0 + G1], x3 = s_pxl[g0 + G1 + 1];
     const float d0 = s_dep[g0], d1 = s_dep[g0 + 1], d2 = s_dep[g0 + G1], d3 = s_dep[g0 + G1 + 1];
-    const bool allnormal = (lane_mask(p0.z == 0.0f) | lane_mask(p1.z == 0.0f) | lane_mask(p2.z == 0.0f) |
-                            lane_mask(p3.z == 0.0f)) == 0;
+    const bool allnormal = (lane_mask(z0 == 0.0f) | lane_mask(z1 == 0.0f) | lane_mask(z2 == 0.0f) |
+                            lane_mask(z3 == 0.0f)) == 0;
     const int32_t vmaxx = (vw.width << sh) - 1, vmaxy = (vw.height << sh) - 1;
     bool ff = true;
     if (prm.backface_culling) {
@@ -2230,90 +2229,106 @@ MP_DEV void shade_sample(const PT* s_pos, const int2* s_pxl, const float* s_dep,
                            lane_mask(qmaxy - qminy < (1 << 11));
     const bool exact = small == __builtin_amdgcn_read_exec();
 
-    // The Lambert term is read back (resolve, adjoint) only for micropolygons
-    // that own a pixel: none in a block whose box the reference empties, and
-    // in the exact-integer regime none that is not front-facing.  The other
-    // entries of the store are left as they were.
-    if (allnormal && (ff || !exact)) quad_store[(size_t)r * Q + (u + v * N)] = quad_lambert(p0, p1, p2, p3);
-
-    // ---- sample (reyes.cl:270-398) ----
-    if (!allnormal) return;   // the reference empties the block's box
-    const uint32_t id = (r - vw.range_base) * (uint32_t)Q + (uint32_t)(u + v * N);
-    unsigned long long* keys = vw.keys;
-    const int gw = vw.grid_w;
-    // pixel-candidate bounds with the reference's initialisers
-    const int32_t mingx = min(vmaxx + 1, qminx), mingy = min(vmaxy + 1, qminy);
-    const int32_t maxgx = max(-1, qmaxx), maxgy = max(-1, qmaxy);
-    if (exact) {
-        if (ff) {
-            // The block's tile box (the wave's clamped quad boxes) only
-            // narrows a front-facing quad's pixels to the viewport and the
-            // tile after its last column / row: pixel X samples at its corner
-            // X << sh, inside the box only for X >= ceil(mingx / 2^sh) (the
-            // reference's loop starts at the floor, whose corner lies outside
-            // the box unless mingx is a whole pixel), from column 0, up to
-            // maxgx >> sh and the last column of the viewport's last tile.
-            // The box contains every front-facing quad's clamped box, so no
-            // wave reduction is needed here.
-            const int32_t rnd = (1 << sh) - 1;
-            const int32_t X0 = max((mingx + rnd) >> sh, 0), X1 = min(maxgx >> sh, (vw.width - 1) | 7);
-            const int32_t Y0 = max((mingy + rnd) >> sh, 0), Y1 = min(maxgy >> sh, (vw.height - 1) | 7);
-            SmallTri t1, t2;
-            small_tri_setup(t1, x0.x, x0.y, x1.x, x1.y, x3.x, x3.y);
-            small_tri_setup(t2, x0.x, x0.y, x3.x, x3.y, x2.x, x2.y);
-            for (int32_t Y = Y0; Y <= Y1; ++Y) {
-                for (int32_t X = X0; X <= X1; ++X) {
-                    const int32_t tx = X << sh, ty = Y << sh;
-                    float dd = 1.0f;
-                    const bool in1 = small_tri_test(t1, tx, ty, d0, d1, d3, dd);
-                    const bool in2 = small_tri_test(t2, tx, ty, d0, d3, d2, dd);
-                    if (in1 || in2) {
-                        const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
-                        key_min(&keys[fb_index(X, Y, gw)], key);
-                        ++natom;
+    // ---- sample (reyes.cl:270-398) ----; true for a quad that covers a sample
+    auto sample = [&]() -> bool {
+        bool hit = false;
+        const uint32_t id = (r - vw.range_base) * (uint32_t)Q + (uint32_t)(u + v * N);
+        unsigned long long* keys = vw.keys;
+        const int gw = vw.grid_w;
+        // pixel-candidate bounds with the reference's initialisers
+        const int32_t mingx = min(vmaxx + 1, qminx), mingy = min(vmaxy + 1, qminy);
+        const int32_t maxgx = max(-1, qmaxx), maxgy = max(-1, qmaxy);
+        if (exact) {
+            if (ff) {
+                // The block's tile box (the wave's clamped quad boxes) only
+                // narrows a front-facing quad's pixels to the viewport and the
+                // tile after its last column / row: pixel X samples at its corner
+                // X << sh, inside the box only for X >= ceil(mingx / 2^sh) (the
+                // reference's loop starts at the floor, whose corner lies outside
+                // the box unless mingx is a whole pixel), from column 0, up to
+                // maxgx >> sh and the last column of the viewport's last tile.
+                // The box contains every front-facing quad's clamped box, so no
+                // wave reduction is needed here.
+                const int32_t rnd = (1 << sh) - 1;
+                const int32_t X0 = max((mingx + rnd) >> sh, 0), X1 = min(maxgx >> sh, (vw.width - 1) | 7);
+                const int32_t Y0 = max((mingy + rnd) >> sh, 0), Y1 = min(maxgy >> sh, (vw.height - 1) | 7);
+                SmallTri t1, t2;
+                small_tri_setup(t1, x0.x, x0.y, x1.x, x1.y, x3.x, x3.y);
+                small_tri_setup(t2, x0.x, x0.y, x3.x, x3.y, x2.x, x2.y);
+                for (int32_t Y = Y0; Y <= Y1; ++Y) {
+                    for (int32_t X = X0; X <= X1; ++X) {
+                        const int32_t tx = X << sh, ty = Y << sh;
+                        float dd = 1.0f;
+                        const bool in1 = small_tri_test(t1, tx, ty, d0, d1, d3, dd);
+                        const bool in2 = small_tri_test(t2, tx, ty, d0, d3, d2, dd);
+                        if (in1 || in2) {
+                            hit = true;
+                            const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
+                            key_min(&keys[fb_index(X, Y, gw)], key);
+                            ++natom;
+                        }
                     }
                 }
             }
+            return hit;
         }
-        return;
-    }
-    // the block's tile box: front-facing quads' boxes, clamped to the viewport
-    int32_t xmin = wave_min_i(ff ? min(vmaxx, qminx) : vmaxx);
-    int32_t ymin = wave_min_i(ff ? min(vmaxy, qminy) : vmaxy);
-    int32_t xmax = wave_max_i(ff ? max(0, qmaxx) : 0);
-    int32_t ymax = wave_max_i(ff ? max(0, qmaxy) : 0);
-    xmin = max(0, xmin); ymin = max(0, ymin);
-    xmax = min(vmaxx, xmax); ymax = min(vmaxy, ymax);
-    const int32_t mintx = xmin >> (sh + 3), minty = ymin >> (sh + 3);
-    const int32_t maxtx = xmax >> (sh + 3), maxty = ymax >> (sh + 3);
-    if (mintx <= maxtx && minty <= maxty) {
-        {
-            // literal per-tile loops of the reference (wide coordinates)
-            if (lane == 0) ++slow;
-            const int32_t maxl = (8 << sh) - 1;
-            for (int32_t ty = minty; ty <= maxty; ++ty) {
-                for (int32_t tx = mintx; tx <= maxtx; ++tx) {
-                    const int32_t osx = (tx * 8) << sh, osy = (ty * 8) << sh;
-                    const int32_t minpx = min(max(wsub(mingx, osx), 0), maxl) >> sh;
-                    const int32_t minpy = min(max(wsub(mingy, osy), 0), maxl) >> sh;
-                    const int32_t maxpx = min(max(wsub(maxgx, osx), 0), maxl) >> sh;
-                    const int32_t maxpy = min(max(wsub(maxgy, osy), 0), maxl) >> sh;
-                    for (int32_t y = minpy; y <= maxpy; ++y) {
-                        for (int32_t x = minpx; x <= maxpx; ++x) {
-                            const int32_t tpx = (x << sh) + osx, tpy = (y << sh) + osy;
-                            float dd = 1.0f;
-                            const bool in1 = inside_tri(x0.x, x0.y, x1.x, x1.y, x3.x, x3.y, d0, d1, d3, tpx, tpy, dd);
-                            const bool in2 = inside_tri(x0.x, x0.y, x3.x, x3.y, x2.x, x2.y, d0, d3, d2, tpx, tpy, dd);
-                            if (in1 || in2) {
-                                const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
-                                key_min(&keys[fb_index(tx * 8 + x, ty * 8 + y, gw)], key);
-                                ++natom;
+        // the block's tile box: front-facing quads' boxes, clamped to the viewport
+        int32_t xmin = wave_min_i(ff ? min(vmaxx, qminx) : vmaxx);
+        int32_t ymin = wave_min_i(ff ? min(vmaxy, qminy) : vmaxy);
+        int32_t xmax = wave_max_i(ff ? max(0, qmaxx) : 0);
+        int32_t ymax = wave_max_i(ff ? max(0, qmaxy) : 0);
+        xmin = max(0, xmin); ymin = max(0, ymin);
+        xmax = min(vmaxx, xmax); ymax = min(vmaxy, ymax);
+        const int32_t mintx = xmin >> (sh + 3), minty = ymin >> (sh + 3);
+        const int32_t maxtx = xmax >> (sh + 3), maxty = ymax >> (sh + 3);
+        if (mintx <= maxtx && minty <= maxty) {
+            {
+                // literal per-tile loops of the reference (wide coordinates)
+                if (lane == 0) ++slow;
+                const int32_t maxl = (8 << sh) - 1;
+                for (int32_t ty = minty; ty <= maxty; ++ty) {
+                    for (int32_t tx = mintx; tx <= maxtx; ++tx) {
+                        const int32_t osx = (tx * 8) << sh, osy = (ty * 8) << sh;
+                        const int32_t minpx = min(max(wsub(mingx, osx), 0), maxl) >> sh;
+                        const int32_t minpy = min(max(wsub(mingy, osy), 0), maxl) >> sh;
+                        const int32_t maxpx = min(max(wsub(maxgx, osx), 0), maxl) >> sh;
+                        const int32_t maxpy = min(max(wsub(maxgy, osy), 0), maxl) >> sh;
+                        for (int32_t y = minpy; y <= maxpy; ++y) {
+                            for (int32_t x = minpx; x <= maxpx; ++x) {
+                                const int32_t tpx = (x << sh) + osx, tpy = (y << sh) + osy;
+                                float dd = 1.0f;
+                                const bool in1 = inside_tri(x0.x, x0.y, x1.x, x1.y, x3.x, x3.y, d0, d1, d3, tpx, tpy, dd);
+                                const bool in2 = inside_tri(x0.x, x0.y, x3.x, x3.y, x2.x, x2.y, d0, d3, d2, tpx, tpy, dd);
+                                if (in1 || in2) {
+                                    hit = true;
+                                    const unsigned long long key = ((unsigned long long)mono_bits(dd) << 32) | id;
+                                    key_min(&keys[fb_index(tx * 8 + x, ty * 8 + y, gw)], key);
+                                    ++natom;
+                                }
                             }
                         }
                     }
                 }
             }
         }
+        return hit;
+    };
+    // The reference empties the box of a block with a zero eye depth.  The
+    // Lambert term is read back (resolve, adjoint) only for micropolygons
+    // that own a pixel.  8x8 grids (small ranges, many quads without a
+    // sample) shade after sampling, only the quads that cover a sample;
+    // larger grids, whose quads nearly all cover one, shade first every quad
+    // that may (front-facing in the exact-integer regime).  The other entries
+    // of the store are left as they were.
+    const size_t qs = (size_t)r * Q + (u + v * N);
+    auto shade = [&]() {
+        return quad_lambert(eye4(s_pos[g0]), eye4(s_pos[g0 + 1]), eye4(s_pos[g0 + G1]), eye4(s_pos[g0 + G1 + 1]));
+    };
+    if constexpr (N == 8) {
+        if (allnormal && sample()) quad_store[qs] = shade();
+    } else {
+        if (allnormal && (ff || !exact)) quad_store[qs] = shade();
+        if (allnormal) sample();
     }
 }
 
