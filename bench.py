@@ -169,6 +169,7 @@ def measure_batch(steps, use_graph):
     tuning = r.autotune(3) if use_graph else r.tuning()
     r.replay(1, use_graph=use_graph)
     batch_ms = r.replay(steps, use_graph=use_graph)
+    stages = {k: round(v, 4) for k, v in r.replay_stages(3).items()}
     r.close()
     print("batch: 64 plain frames %.3f ms, one batched frame %.3f ms" % (plain_ms, batch_ms),
           file=sys.stderr, flush=True)
@@ -177,6 +178,7 @@ def measure_batch(steps, use_graph):
             "plain_frames_autotuned": bool(use_graph),
             "batched_ms": round(batch_ms, 4), "speedup": round(plain_ms / batch_ms, 3),
             "batched_tuning": {k: tuning[k] for k in TUNING_KEYS},
+            "batched_stages_ms": stages, "batched_ranges": int(st["diced_ranges"]),
             "same_micropolygons": int(mpolys) == int(st["micropolygons"])}
 
 
