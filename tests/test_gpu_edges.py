@@ -98,8 +98,11 @@ def test_ragged_windows(oracle, size):
     r.close()
 
 
-def test_mixed_objects_and_instancing(oracle):
-    """Bezier torus drawn twice (same patches, two matrices), Gregory planes."""
+@pytest.mark.parametrize("n", [16, 8])
+def test_mixed_objects_and_instancing(oracle, n):
+    """Bezier torus drawn twice (same patches, two matrices), Gregory planes;
+    at N = 8 the raster dices two consecutive ranges together only when they
+    share a Bezier object, checked under every range schedule."""
     t = scenes.c2_torus(width=480, height=270)
     g = scenes.c4_gregory(planes=2, quads=32, width=480, height=270)
     tor = t.views[0].objects[0]
@@ -111,10 +114,18 @@ def test_mixed_objects_and_instancing(oracle):
     greg = [scenes.SceneObject(100 + k, o.patches, o.patch_type, o.matrix, o.color)
             for k, o in enumerate(g.views[0].objects)]
     objs = [tor, twin] + greg
-    w = scenes.Workload("mixed", dict(LOCAL, displacement=0),
-                        [scenes.View(t.views[0].projection, objs)])
-    r, got, ref = _render_both(oracle, w, "mixed")
+    opts = dict(LOCAL, displacement=0, reyes_patch_size=n, bound_n_split_limit=float(n))
+    w = scenes.Workload("mixed", opts, [scenes.View(t.views[0].projection, objs)])
+    r, got, ref = _render_both(oracle, w, "mixed N=%d" % n)
     assert got["stats"]["diced_ranges"] == len(ref["ranges"])
+    for chunk in (0, -1, 5, -16):
+        t = {k: v for k, v in r.tuning().items() if k != "frame_ms"}
+        r.set_tuning(**dict(t, raster_chunk=chunk))
+        r.replay(2, use_graph=True)
+        again = r.read_framebuffer()
+        assert np.array_equal(again["ids"], got["ids"]), chunk
+        assert np.array_equal(again["depth"], got["depth"]), chunk
+        assert np.array_equal(again["rgba8"], got["rgba8"]), chunk
     r.close()
 
 
