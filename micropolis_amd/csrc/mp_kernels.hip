@@ -1820,6 +1820,22 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place_ranked(const DevObject*
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const uint32_t g = blockIdx.x;
     const uint32_t cnt = min(cb.forest_total[g], (uint32_t)kForestKeys);
+    // the group's records and ranks load now, in flight during the scan
+    // (read only once the frame is known whole)
+    constexpr uint32_t PR = (kForestKeys + kPlaceThreads - 1) / kPlaceThreads;
+    // the records as six words each (a RangeRec array would live in scratch)
+    uint32_t w[PR][6], rk[PR];
+#pragma unroll
+    for (uint32_t j = 0; j < PR; ++j) {
+        const uint32_t i = t + j * kPlaceThreads;
+        if (i < cnt) {
+            const size_t e = (size_t)g * kForestKeys + i;
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(cb.forest_out + e);
+#pragma unroll
+            for (int f = 0; f < 6; ++f) w[j][f] = src[f];
+            rk[j] = cb.rank[e];
+        }
+    }
     uint32_t c[PER], sum = 0;
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
@@ -1876,11 +1892,17 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place_ranked(const DevObject*
         }
     }
     if (!whole) return;
-    for (uint32_t i = t; i < cnt; i += kPlaceThreads) {
-        const size_t e = (size_t)g * kForestKeys + i;
-        const RangeRec r = cb.forest_out[e];
-        const uint32_t slot = s_off[gpid_of(objs, r)] + cb.rank[e];
-        if (slot < cb.cap) cb.canon[slot] = r;
+#pragma unroll
+    for (uint32_t j = 0; j < PR; ++j) {
+        if (t + j * kPlaceThreads < cnt) {
+            // gpid_of: the object's first global patch index + pid
+            const uint32_t slot = s_off[objs[w[j][1] & 0xffffffu].gpid_base + w[j][0]] + rk[j];
+            if (slot < cb.cap) {
+                uint32_t* dst = reinterpret_cast<uint32_t*>(cb.canon + slot);
+#pragma unroll
+                for (int f = 0; f < 6; ++f) dst[f] = w[j][f];
+            }
+        }
     }
 }
 
