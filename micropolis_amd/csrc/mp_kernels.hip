@@ -2469,16 +2469,17 @@ MP_DEV void dice_grid(const float4* __restrict__ s_cp, float* __restrict__ s_col
     }
 }
 
-// Undisplaced Bezier dice (dice.cl:38-79) of range a, or of ranges a and b
-// of one object (two), by one wave into the flat grid arrays: a's points at
-// [0, G), b's at [G, 2G).  A grid of 64 < G <= 96 points takes two lane
-// passes per range alone; two ranges take three: a's and b's first 64
+// Undisplaced dice (dice.cl:38-79) of range a, or of ranges a and b of one
+// object (two), by one wave into the flat grid arrays: a's points at
+// [0, G), b's at [G, 2G).  A grid of 64 < G <= 96 points takes two points per
+// lane (one pass, a Bezier patch's control points read once for both) per
+// range alone; two ranges take three lane passes: a's and b's first 64
 // points side by side, then the G - 64 left of each, a's on lanes 0..31 and
 // b's on lanes 32..63.
 template <int N, typename PT>
-MP_DEV void dice_bezier_pair(const float4* cpa, const float4* cpb, const RangeRec& ra, const RangeRec& rb,
-                             bool two, const DevObject& ob, const DevView& vw,
-                             PT* __restrict__ s_pos, int2* __restrict__ s_pxl, float* __restrict__ s_dep)
+MP_DEV void dice_pair(const float4* cpa, const float4* cpb, const RangeRec& ra, const RangeRec& rb, bool two,
+                      const DevObject& ob, const DevView& vw, PT* __restrict__ s_pos, int2* __restrict__ s_pxl,
+                      float* __restrict__ s_dep)
 {
     constexpr int G1 = N + 1;
     constexpr int G = G1 * G1;
@@ -2500,20 +2501,26 @@ MP_DEV void dice_bezier_pair(const float4* cpa, const float4* cpb, const RangeRe
     // umin + (umax - umin) * i / N, fused as in dice_grid
     const float dua = ra.umax - ra.umin, dva = ra.vmax - ra.vmin;
     const float dub = rb.umax - rb.umin, dvb = rb.vmax - rb.vmin;
+    const bool greg = ob.type == MP_TYPE_GREGORY;
     float P0[4], P1[4];
+    auto eval2 = [&](const float4* c0, const float4* c1, float u0, float v0, float u1, float v1) {
+        if (greg) {
+            eval_gregory(c0, u0, v0, P0);
+            eval_gregory(c1, u1, v1, P1);
+        } else {
+            eval_bezier2(c0, c1, u0, v0, u1, v1, P0, P1);
+        }
+    };
     if (!two) {
         const int g1 = min(64 + lane, G - 1);
-        eval_bezier2(cpa, cpa, mad(dua, (float)(lane % G1) / (float)N, ra.umin),
-                     mad(dva, (float)(lane / G1) / (float)N, ra.vmin),
-                     mad(dua, (float)(g1 % G1) / (float)N, ra.umin),
-                     mad(dva, (float)(g1 / G1) / (float)N, ra.vmin), P0, P1);
+        eval2(cpa, cpa, mad(dua, (float)(lane % G1) / (float)N, ra.umin), mad(dva, (float)(lane / G1) / (float)N, ra.vmin),
+              mad(dua, (float)(g1 % G1) / (float)N, ra.umin), mad(dva, (float)(g1 / G1) / (float)N, ra.vmin));
         project(lane, P0, true);
         project(g1, P1, 64 + lane < G);
         return;
     }
     const float tu = (float)(lane % G1) / (float)N, tv = (float)(lane / G1) / (float)N;
-    eval_bezier2(cpa, cpb, mad(dua, tu, ra.umin), mad(dva, tv, ra.vmin), mad(dub, tu, rb.umin),
-                 mad(dvb, tv, rb.vmin), P0, P1);
+    eval2(cpa, cpb, mad(dua, tu, ra.umin), mad(dva, tv, ra.vmin), mad(dub, tu, rb.umin), mad(dvb, tv, rb.vmin));
     project(lane, P0, true);
     project(G + lane, P1, true);
     const bool hb = lane >= 32;
@@ -2521,8 +2528,9 @@ MP_DEV void dice_bezier_pair(const float4* cpa, const float4* cpb, const RangeRe
     const int g = 64 + min(k, G - 65);
     const float du = hb ? dub : dua, dv = hb ? dvb : dva;
     const float umin = hb ? rb.umin : ra.umin, vmin = hb ? rb.vmin : ra.vmin;
-    eval_bezier(hb ? cpb : cpa, mad(du, (float)(g % G1) / (float)N, umin), mad(dv, (float)(g / G1) / (float)N, vmin),
-                P0);
+    const float u = mad(du, (float)(g % G1) / (float)N, umin), v = mad(dv, (float)(g / G1) / (float)N, vmin);
+    if (greg) eval_gregory(hb ? cpb : cpa, u, v, P0);
+    else eval_bezier(hb ? cpb : cpa, u, v, P0);
     project((hb ? G : 0) + g, P0, k < G - 64);
 }
 
@@ -2542,7 +2550,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     constexpr int Q = N * N;
     constexpr int B = N / 8;
     // One-wave work-groups of 9 x 9 undisplaced grids dice two ranges of one
-    // Bezier object together (three lane passes instead of four).
+    // object together (three lane passes instead of four).
     constexpr bool kPair = MODE == 0 && NT == 64 && G > 64 && 2 * (G - 64) <= 64;
     constexpr int GS = kPair ? 2 * G : G;
     __shared__ float4 s_cpb[kPair ? 1 : 2][20];   // control points: current range / next range
@@ -2644,7 +2652,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             bool two = false;
             if (b < total) {
                 rb = canon[b];
-                two = (rb.obj_depth & 0xffffffu) == oa && ob.type == MP_TYPE_BEZIER;
+                two = (rb.obj_depth & 0xffffffu) == oa;
             }
             // the next window: c after this one or two, d after c; their
             // control points load now and go to the other buffer after the dice
@@ -2654,10 +2662,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (c < total) next(d, d_end);
             else d = total;
             const bool have_pf = fetch_two(c, d, pf);
-            if (ob.type == MP_TYPE_BEZIER)
-                dice_bezier_pair<N>(s_cq[qb][0], s_cq[qb][1], ra, rb, two, ob, vw, s_pos, s_pxl, s_dep);
-            else
-                dice_grid<N, 0, NT>(s_cq[qb][0], s_col, s_wt, ra, ob, vw, prm, s_pos, s_pxl, s_dep, nw);
+            dice_pair<N>(s_cq[qb][0], s_cq[qb][1], ra, rb, two, ob, vw, s_pos, s_pxl, s_dep);
             __syncthreads();
             if (have_pf) s_cq[qb ^ 1][lane >> 5][lane & 31] = pf;
             // one shading code path for both ranges (instruction cache)
