@@ -347,7 +347,6 @@ DevParams mp::device_params(const mp_config* cfg)
     p.disp_frequency = cfg->displacement_frequency;
     p.disp_seed = cfg->displacement_seed;
     p.disp_octaves = cfg->displacement_octaves;
-    p.noise_cache = 0;        // hashing the lattice per use measured faster than the per-range LDS cache
     p.noise_wave_cells = 2;   // lane window (0 per lane, 1 wave-uniform cell)
     p.raster_chunk = 0;       // static range stride (see mp_tuning)
     p.raster_one_wave = 1;    // N = 16: one wave per work-group (no waits between waves)
@@ -458,7 +457,6 @@ extern "C" mp_context* mp_create(const mp_config* cfg)
         }
     }
     ctx->num_cus = prop.multiProcessorCount;
-    if (const char* e = getenv("MP_NOISE_CACHE")) ctx->prm.noise_cache = atoi(e);
     if (const char* e = getenv("MP_RASTER_ONE_WAVE")) ctx->prm.raster_one_wave = atoi(e) != 0;
     if (const char* e = getenv("MP_RASTER_CHUNK")) ctx->prm.raster_chunk = atoi(e);
     if (const char* e = getenv("MP_NOISE_WAVE_CELLS")) ctx->prm.noise_wave_cells = atoi(e);

@@ -587,41 +587,6 @@ MP_DEV void bezier_unit_normal(const float4* __restrict__ cp, float u, float v, 
     normal_from_columns(col, v, n);
 }
 
-// Lattice values of one noise octave cached for a box of lattice cells.
-struct LatticeCache {
-    const float* v;
-    int lx, ly, lz;      // lattice origin
-    int nx, ny, nz;      // extent (0 = not cached)
-};
-
-// vnoise with the corner lattice values taken from the cache when the cell
-// lies inside it (same values as hashing them).
-MP_DEV float vnoise_cached(float x, float y, float z, uint32_t seed, const LatticeCache& L)
-{
-    float fx = floorf(x), fy = floorf(y), fz = floorf(z);
-    int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
-    float wx = fade(x - fx), wy = fade(y - fy), wz = fade(z - fz);
-    const int ox = ix - L.lx, oy = iy - L.ly, oz = iz - L.lz;
-    float c000, c100, c010, c110, c001, c101, c011, c111;
-    if (ox >= 0 && oy >= 0 && oz >= 0 && ox + 1 < L.nx && oy + 1 < L.ny && oz + 1 < L.nz) {
-        const int sy = L.nx, sz = L.nx * L.ny;
-        const float* b = L.v + (oz * sz + oy * sy + ox);
-        c000 = b[0];       c100 = b[1];
-        c010 = b[sy];      c110 = b[sy + 1];
-        c001 = b[sz];      c101 = b[sz + 1];
-        c011 = b[sz + sy]; c111 = b[sz + sy + 1];
-    } else {
-        c000 = lattice(ix, iy, iz, seed);         c100 = lattice(ix + 1, iy, iz, seed);
-        c010 = lattice(ix, iy + 1, iz, seed);     c110 = lattice(ix + 1, iy + 1, iz, seed);
-        c001 = lattice(ix, iy, iz + 1, seed);     c101 = lattice(ix + 1, iy, iz + 1, seed);
-        c011 = lattice(ix, iy + 1, iz + 1, seed); c111 = lattice(ix + 1, iy + 1, iz + 1, seed);
-    }
-    float x00 = lerpf_(c000, c100, wx), x10 = lerpf_(c010, c110, wx);
-    float x01 = lerpf_(c001, c101, wx), x11 = lerpf_(c011, c111, wx);
-    float y0 = lerpf_(x00, x10, wy), y1 = lerpf_(x01, x11, wy);
-    return lerpf_(y0, y1, wz);
-}
-
 // Displaced object-space point P + amp*noise(P)*unit(cross(dP/dv, dP/du)).
 MP_DEV void displaced_point(const float4* __restrict__ cp, int type, float u, float v,
                             const DevParams& prm, float amp, float out[4])

@@ -82,7 +82,6 @@ MP_DEV void reset_counters(FrameCounters* fc, ShareState* share)
     fc->big_ranges = 0;
     fc->atomics = 0;
     fc->raster_next = 0;
-    fc->uncached_octaves = 0;
 }
 
 __global__ void k_reset(FrameCounters* fc, ShareState* share)
@@ -2196,11 +2195,7 @@ MP_DEV float quad_lambert(const float4& p0, const float4& p1, const float4& p2, 
     return (spec != spec) ? spec : s;
 }
 
-constexpr int kCachedOctaves = 4;
-constexpr int kCacheEntries = 512;
-
-// MODE 0: no displacement; 1: displacement, lattice values hashed per use;
-// 2: displacement with the per-range LDS lattice cache.
+// MODE 0: no displacement; 1: displacement (value noise by lane windows).
 // Shade (reyes.cl:84-199) and sample (reyes.cl:270-398) one 8x8 block of a
 // diced range: the calling wave's lane (u, v) owns quad (u, v); s_pos /
 // s_pxl / s_dep hold the range's (N+1)^2 grid; r is its canonical index.
@@ -2555,16 +2550,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     constexpr int GS = kPair ? 2 * G : G;
     __shared__ float4 s_cpb[kPair ? 1 : 2][20];   // control points: current range / next range
     __shared__ float4 s_cq[kPair ? 2 : 1][2][20];  // pairs: [buffer][first, second range]
-    // eye-space x, y, z per grid point (the LDS-cached noise path also keeps
-    // the object-space w there between its passes)
-    typedef typename std::conditional<MODE == 2, float4, EyePos>::type PosT;
-    __shared__ PosT s_pos[GS];
+    // eye-space x, y, z per grid point
+    __shared__ EyePos s_pos[GS];
     __shared__ int2 s_pxl[GS];
     __shared__ float s_dep[GS];
-    __shared__ float4 s_nrm[G];
-    __shared__ float s_box[NT / 64][6];
-    __shared__ int s_geo[kCachedOctaves][6];
-    __shared__ float s_lat[kCachedOctaves][kCacheEntries];
     __shared__ float s_col[MODE == 1 ? 3 * 8 * G1 : 1];   // normal column sums per u column
     __shared__ float4 s_wt[MODE == 1 ? 3 * G1 : 1];       // Bernstein weights per u column / v row
     __shared__ uint4 s_win[NT / 64][kKeptOctaves];         // kept lattice windows per wave
@@ -2573,7 +2562,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int wave = tid >> 6, lane = tid & 63;
     // the canonical ranges [first, first + count), or all of the frame's (count 0)
     const uint32_t total = count ? first + count : fc->total_ranges;
-    uint32_t slow = 0, natom = 0, uncached = 0;
+    uint32_t slow = 0, natom = 0;
     __shared__ uint32_t s_next;
 
     // Range schedule.  Static (prm.raster_chunk == 0): work-group g takes
@@ -2718,104 +2707,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const DevView& vw = views[ob.view];
 
         // ---- dice (dice.cl:38-79) ----
-        if (MODE != 2 || ob.type != MP_TYPE_BEZIER) {
-            dice_grid<N, MODE == 2 ? 0 : MODE, NT>(s_cp, s_col, s_wt, rec, ob, vw, prm, s_pos, s_pxl, s_dep, nw);
-        } else if constexpr (MODE == 2) {
-            // displaced: surface point and normal, then the value noise with
-            // each octave's lattice values hashed once per range into LDS
-            float bx0 = INFINITY, by0 = INFINITY, bz0 = INFINITY;
-            float bx1 = -INFINITY, by1 = -INFINITY, bz1 = -INFINITY;
-            for (int g = tid; g < G; g += NT) {
-                const int nu = g % G1, nv = g / G1;
-                const float tu = (float)nu / (float)N, tv = (float)nv / (float)N;
-                const float uu = rec.umin + (rec.umax - rec.umin) * tu;
-                const float vv = rec.vmin + (rec.vmax - rec.vmin) * tv;
-                float P[4], n[3];
-                eval_bezier(s_cp, uu, vv, P);
-                bezier_unit_normal(s_cp, uu, vv, n);
-                s_pos[g] = make_float4(P[0], P[1], P[2], P[3]);
-                s_nrm[g] = make_float4(n[0], n[1], n[2], 0.0f);
-                bx0 = fminf(bx0, P[0]); by0 = fminf(by0, P[1]); bz0 = fminf(bz0, P[2]);
-                bx1 = fmaxf(bx1, P[0]); by1 = fmaxf(by1, P[1]); bz1 = fmaxf(bz1, P[2]);
-            }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                bx0 = fminf(bx0, __shfl_xor(bx0, o)); by0 = fminf(by0, __shfl_xor(by0, o));
-                bz0 = fminf(bz0, __shfl_xor(bz0, o)); bx1 = fmaxf(bx1, __shfl_xor(bx1, o));
-                by1 = fmaxf(by1, __shfl_xor(by1, o)); bz1 = fmaxf(bz1, __shfl_xor(bz1, o));
-            }
-            if (lane == 0) {
-                s_box[wave][0] = bx0; s_box[wave][1] = by0; s_box[wave][2] = bz0;
-                s_box[wave][3] = bx1; s_box[wave][4] = by1; s_box[wave][5] = bz1;
-            }
-            __syncthreads();
-            const int noct = min(prm.disp_octaves, kCachedOctaves);
-            if (tid < noct) {
-                float m[6];
-                for (int k = 0; k < 6; ++k) m[k] = s_box[0][k];
-                for (int w = 1; w < NT / 64; ++w)
-                    for (int k = 0; k < 3; ++k) {
-                        m[k] = fminf(m[k], s_box[w][k]);
-                        m[k + 3] = fmaxf(m[k + 3], s_box[w][k + 3]);
-                    }
-                float f = prm.disp_frequency;
-                for (int k = 0; k < tid; ++k) f = f * 2.0f;
-                int geo[6] = { 0, 0, 0, 0, 0, 0 };
-                bool ok = true;
-                for (int k = 0; k < 3; ++k) {
-                    const float lo = floorf(m[k] * f), hi = floorf(m[k + 3] * f);
-                    ok = ok && (lo >= -1.0e6f) && (hi <= 1.0e6f) && (hi >= lo);
-                    if (ok) { geo[k] = (int)lo; geo[k + 3] = (int)hi - (int)lo + 2; }
-                }
-                if (!ok || geo[3] > kCacheEntries || geo[4] > kCacheEntries || geo[5] > kCacheEntries ||
-                    geo[3] * geo[4] * geo[5] > kCacheEntries)
-                    geo[3] = geo[4] = geo[5] = 0;
-                for (int k = 0; k < 6; ++k) s_geo[tid][k] = geo[k];
-                if (geo[3] == 0) ++uncached;
-            }
-            __syncthreads();
-            for (int k = 0; k < noct; ++k) {
-                const int nx = s_geo[k][3], ny = s_geo[k][4], nzc = s_geo[k][5];
-                const int cnt = nx * ny * nzc;
-                for (int i = tid; i < cnt; i += NT) {
-                    const int x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
-                    s_lat[k][i] = lattice(s_geo[k][0] + x, s_geo[k][1] + y, s_geo[k][2] + z,
-                                          prm.disp_seed + (uint32_t)k);
-                }
-            }
-            __syncthreads();
-            for (int g = tid; g < G; g += NT) {
-                const float4 p4 = s_pos[g];
-                const float4 n4 = s_nrm[g];
-                float acc = 0.0f, amp = 1.0f, norm = 0.0f, freq = prm.disp_frequency;
-                for (int k = 0; k < prm.disp_octaves; ++k) {
-                    float nv;
-                    if (k < kCachedOctaves) {
-                        LatticeCache L;
-                        L.v = s_lat[k];
-                        L.lx = s_geo[k][0]; L.ly = s_geo[k][1]; L.lz = s_geo[k][2];
-                        L.nx = s_geo[k][3]; L.ny = s_geo[k][4]; L.nz = s_geo[k][5];
-                        nv = vnoise_cached(p4.x * freq, p4.y * freq, p4.z * freq, prm.disp_seed + (uint32_t)k, L);
-                    } else {
-                        nv = vnoise(p4.x * freq, p4.y * freq, p4.z * freq, prm.disp_seed + (uint32_t)k);
-                    }
-                    acc = mad(amp, nv, acc);
-                    norm = norm + amp;
-                    amp = amp * 0.5f;
-                    freq = freq * 2.0f;
-                }
-                const float d = ob.disp_amplitude * (acc / norm);
-                float P[4] = { mad(d, n4.x, p4.x), mad(d, n4.y, p4.y), mad(d, n4.z, p4.z), p4.w };
-                float e[4], q[4];
-                mat4_mul(ob.mv, P, e);
-                mat4_mul(vw.proj, e, q);
-                set_pos(s_pos[g], e[0], e[1], e[2], e[3]);
-                const int32_t px = f2i_sat((q[0] / q[3]) * (float)vw.vs_x / 2.0f + (float)(vw.vs_x / 2));
-                const int32_t py = f2i_sat((q[1] / q[3]) * (float)vw.vs_y / 2.0f + (float)(vw.vs_y / 2));
-                s_pxl[g] = make_int2(px, py);
-                s_dep[g] = q[2] / q[3];
-            }
-        }
+        dice_grid<N, MODE, NT>(s_cp, s_col, s_wt, rec, ob, vw, prm, s_pos, s_pxl, s_dep, nw);
         __syncthreads();
 
         // The next range's control points go to the other buffer now: the
@@ -2837,7 +2729,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         __syncthreads();
     }
     if (slow) atomicAdd(&fc->slow_blocks, slow);
-    if (uncached) atomicAdd(&fc->uncached_octaves, uncached);
     natom = (uint32_t)wave_sum_i((int)natom);
     if (lane == 0 && natom) atomicAdd(stripe(fc, blockIdx.x, kStripeAtomics), (unsigned long long)natom);
 }
@@ -2853,7 +2744,7 @@ static int raster_occupancy()
 
 static int raster_mode(const DevParams& prm)
 {
-    return prm.displacement ? (prm.noise_cache ? 2 : 1) : 0;
+    return prm.displacement ? 1 : 0;
 }
 
 int raster_grid(int patch_size, int num_cus, const DevParams& prm)
@@ -2861,8 +2752,7 @@ int raster_grid(int patch_size, int num_cus, const DevParams& prm)
     const int mode = raster_mode(prm);
     int per_cu = 1;
 #define MP_OCC(NN, NT)                                                                     \
-    per_cu = mode == 0 ? raster_occupancy<NN, 0, NT>() : (mode == 1 ? raster_occupancy<NN, 1, NT>() \
-                                                                    : raster_occupancy<NN, 2, NT>());
+    per_cu = mode == 0 ? raster_occupancy<NN, 0, NT>() : raster_occupancy<NN, 1, NT>();
     switch (patch_size) {
     case 8: MP_OCC(8, 64) break;
     case 16:
@@ -2883,11 +2773,8 @@ static void launch_raster_n(int mode, int grid, hipStream_t s, const RangeRec* c
     if (mode == 0)
         hipLaunchKernelGGL((k_raster<N, 0, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store,
                            first, count);
-    else if (mode == 1)
-        hipLaunchKernelGGL((k_raster<N, 1, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store,
-                           first, count);
     else
-        hipLaunchKernelGGL((k_raster<N, 2, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store,
+        hipLaunchKernelGGL((k_raster<N, 1, NT>), dim3(grid), dim3(NT), 0, s, canon, objs, views, prm, fc, quad_store,
                            first, count);
 }
 

@@ -74,7 +74,6 @@ struct DevParams {
     float disp_frequency;
     uint32_t disp_seed;
     int32_t disp_octaves;
-    int32_t noise_cache;         // cache lattice values per range in LDS
     int32_t noise_wave_cells;    // hash wave-uniform lattice cells once per wave
     int32_t raster_one_wave;     // N = 16: one-wave raster work-groups
     int32_t raster_chunk;        // 0: static range stride; -1: dynamic, sized
@@ -93,7 +92,6 @@ struct FrameCounters {
     uint32_t big_ranges;         // ranges of buckets ordered by the radix sort
     unsigned long long atomics;  // (micropolygon, pixel) coverage events = key atomics
     uint32_t raster_next;        // next canonical range to rasterise
-    uint32_t uncached_octaves;   // noise octaves evaluated by hashing
 };
 
 // The counters every work-group of a large launch adds to (coverage samples,
