@@ -2680,17 +2680,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // next range (the following chunk is claimed one range ahead) and its
         // control points: loaded now, stored to the other LDS buffer after
         // this range, so the load latency hides behind this range's work
-        uint32_t rn = (dyn || blk) ? r + 1 : r + gridDim.x, rn_end = r_end;
-        if (dyn && rn >= r_end) {
-            if (tid == 0) s_next = atomicAdd(&fc->raster_next, chunk);
-            __syncthreads();
-            rn = s_next;
-            rn_end = min(rn + chunk, total);
-        } else if (blk && rn >= r_end) {
-            cstart += gridDim.x * chunk;
-            rn = cstart;
-            rn_end = min(cstart + chunk, total);
-        }
+        uint32_t rn = r, rn_end = r_end;
+        next(rn, rn_end);
         bool have_pf = false;
         if (rn < total) {
             const RangeRec nrec = canon[rn];
