@@ -2464,21 +2464,24 @@ MP_DEV void dice_grid(const float4* __restrict__ s_cp, float* __restrict__ s_col
     }
 }
 
-// Undisplaced dice (dice.cl:38-79) of range a, or of ranges a and b of one
-// object (two), by one wave into the flat grid arrays: a's points at
-// [0, G), b's at [G, 2G).  A grid of 64 < G <= 96 points takes two points per
-// lane (one pass, a Bezier patch's control points read once for both) per
-// range alone; two ranges take three lane passes: a's and b's first 64
-// points side by side, then the G - 64 left of each, a's on lanes 0..31 and
-// b's on lanes 32..63.
+// Undisplaced dice (dice.cl:38-79) of the k (1 to 3) consecutive ranges a,
+// b, c of one object by one wave into the flat grid arrays: range j's points
+// at [jG, (j + 1)G).  A grid of 64 < G <= 85 points takes, for one range,
+// one lane pass of two points per lane (a Bezier patch's control points read
+// once for both); for two ranges, a's and b's first 64 points side by side,
+// then the G - 64 left of each (a's on lanes 0..31, b's on 32..63); for
+// three, then c's first 64 points beside the three ranges' last G - 64 (a's
+// on the first lanes, b's after them, c's after those): two passes of two
+// points per lane for three ranges.
 template <int N, typename PT>
-MP_DEV void dice_pair(const float4* cpa, const float4* cpb, const RangeRec& ra, const RangeRec& rb, bool two,
-                      const DevObject& ob, const DevView& vw, PT* __restrict__ s_pos, int2* __restrict__ s_pxl,
-                      float* __restrict__ s_dep)
+MP_DEV void dice_group(const float4* cpa, const float4* cpb, const float4* cpc, const RangeRec& ra,
+                       const RangeRec& rb, const RangeRec& rc, int k, const DevObject& ob, const DevView& vw,
+                       PT* __restrict__ s_pos, int2* __restrict__ s_pxl, float* __restrict__ s_dep)
 {
     constexpr int G1 = N + 1;
     constexpr int G = G1 * G1;
-    static_assert(G > 64 && 2 * (G - 64) <= 64, "grid of 64 < G <= 96 points");
+    constexpr int T = G - 64;   // points past the first 64 of a grid
+    static_assert(G > 64 && 3 * T <= 64, "grid of 64 < G <= 85 points");
     const int lane = threadIdx.x & 63;
     auto project = [&](int f, const float P[4], bool store) {
         float e[4], q[4];
@@ -2496,6 +2499,7 @@ MP_DEV void dice_pair(const float4* cpa, const float4* cpb, const RangeRec& ra, 
     // umin + (umax - umin) * i / N, fused as in dice_grid
     const float dua = ra.umax - ra.umin, dva = ra.vmax - ra.vmin;
     const float dub = rb.umax - rb.umin, dvb = rb.vmax - rb.vmin;
+    const float duc = rc.umax - rc.umin, dvc = rc.vmax - rc.vmin;
     const bool greg = ob.type == MP_TYPE_GREGORY;
     float P0[4], P1[4];
     auto eval2 = [&](const float4* c0, const float4* c1, float u0, float v0, float u1, float v1) {
@@ -2506,7 +2510,7 @@ MP_DEV void dice_pair(const float4* cpa, const float4* cpb, const RangeRec& ra, 
             eval_bezier2(c0, c1, u0, v0, u1, v1, P0, P1);
         }
     };
-    if (!two) {
+    if (k == 1) {
         const int g1 = min(64 + lane, G - 1);
         eval2(cpa, cpa, mad(dua, (float)(lane % G1) / (float)N, ra.umin), mad(dva, (float)(lane / G1) / (float)N, ra.vmin),
               mad(dua, (float)(g1 % G1) / (float)N, ra.umin), mad(dva, (float)(g1 / G1) / (float)N, ra.vmin));
@@ -2518,15 +2522,29 @@ MP_DEV void dice_pair(const float4* cpa, const float4* cpb, const RangeRec& ra, 
     eval2(cpa, cpb, mad(dua, tu, ra.umin), mad(dva, tv, ra.vmin), mad(dub, tu, rb.umin), mad(dvb, tv, rb.vmin));
     project(lane, P0, true);
     project(G + lane, P1, true);
-    const bool hb = lane >= 32;
-    const int k = lane & 31;
-    const int g = 64 + min(k, G - 65);
-    const float du = hb ? dub : dua, dv = hb ? dvb : dva;
-    const float umin = hb ? rb.umin : ra.umin, vmin = hb ? rb.vmin : ra.vmin;
-    const float u = mad(du, (float)(g % G1) / (float)N, umin), v = mad(dv, (float)(g / G1) / (float)N, vmin);
-    if (greg) eval_gregory(hb ? cpb : cpa, u, v, P0);
-    else eval_bezier(hb ? cpb : cpa, u, v, P0);
-    project((hb ? G : 0) + g, P0, k < G - 64);
+    if (k == 2) {
+        const bool hb = lane >= 32;
+        const int j = lane & 31;
+        const int g = 64 + min(j, T - 1);
+        const float du = hb ? dub : dua, dv = hb ? dvb : dva;
+        const float umin = hb ? rb.umin : ra.umin, vmin = hb ? rb.vmin : ra.vmin;
+        const float u = mad(du, (float)(g % G1) / (float)N, umin), v = mad(dv, (float)(g / G1) / (float)N, vmin);
+        if (greg) eval_gregory(hb ? cpb : cpa, u, v, P0);
+        else eval_bezier(hb ? cpb : cpa, u, v, P0);
+        project((hb ? G : 0) + g, P0, j < T);
+        return;
+    }
+    // range t's last T points on lanes tT .. tT + T - 1
+    const int t = lane < T ? 0 : lane < 2 * T ? 1 : 2;
+    const int j = lane - T * t;
+    const int g = 64 + min(j, T - 1);
+    const float du = t == 0 ? dua : t == 1 ? dub : duc, dv = t == 0 ? dva : t == 1 ? dvb : dvc;
+    const float umin = t == 0 ? ra.umin : t == 1 ? rb.umin : rc.umin;
+    const float vmin = t == 0 ? ra.vmin : t == 1 ? rb.vmin : rc.vmin;
+    eval2(cpc, t == 0 ? cpa : t == 1 ? cpb : cpc, mad(duc, tu, rc.umin), mad(dvc, tv, rc.vmin),
+          mad(du, (float)(g % G1) / (float)N, umin), mad(dv, (float)(g / G1) / (float)N, vmin));
+    project(2 * G + lane, P0, true);
+    project(t * G + g, P1, j < T);
 }
 
 // NT threads per work-group: N*N (one wave per 8x8 block of quads) or 64
@@ -2544,12 +2562,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     constexpr int G = G1 * G1;
     constexpr int Q = N * N;
     constexpr int B = N / 8;
-    // One-wave work-groups of 9 x 9 undisplaced grids dice two ranges of one
-    // object together (three lane passes instead of four).
-    constexpr bool kPair = MODE == 0 && NT == 64 && G > 64 && 2 * (G - 64) <= 64;
-    constexpr int GS = kPair ? 2 * G : G;
+    // One-wave work-groups of 9 x 9 undisplaced grids dice up to three
+    // consecutive ranges of one object together (three ranges in two lane
+    // passes of two points per lane, two in three lane passes).
+    constexpr bool kPair = MODE == 0 && NT == 64 && G > 64 && 3 * (G - 64) <= 64;
+    constexpr int GS = kPair ? 3 * G : G;
     __shared__ float4 s_cpb[kPair ? 1 : 2][20];   // control points: current range / next range
-    __shared__ float4 s_cq[kPair ? 2 : 1][2][20];  // pairs: [buffer][first, second range]
+    __shared__ float4 s_cq[kPair ? 2 : 1][3][20];  // groups: [buffer][range of the group]
     // eye-space x, y, z per grid point
     __shared__ EyePos s_pos[GS];
     __shared__ int2 s_pxl[GS];
@@ -2609,59 +2628,81 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         x = xn;
     };
-    // control points of ranges x (lanes 0..31) and y (lanes 32..63) into pf
-    auto fetch_two = [&](uint32_t x, uint32_t y, float4& pf) {
-        const uint32_t z = lane < 32 ? x : y;
-        if (z < total) {
-            const RangeRec rz = canon[z];
-            const DevObject& oz = objs[rz.obj_depth & 0xffffffu];
-            const int ncp = oz.type == MP_TYPE_GREGORY ? 20 : 16;
-            if ((lane & 31) < ncp) {
-                pf = ldg4(oz.cps + (size_t)rz.pid * ncp + (lane & 31));
+    // control points of ranges x (lanes 0..19), y (20..39) and z (40..59)
+    // into pf, for slot lane / 20, entry lane % 20
+    const int fslot = lane / 20, fent = lane - 20 * (lane / 20);
+    auto fetch3 = [&](uint32_t x, uint32_t y, uint32_t z, float4& pf) {
+        const uint32_t w = fslot == 0 ? x : fslot == 1 ? y : fslot == 2 ? z : total;
+        if (w < total) {
+            const RangeRec rw = canon[w];
+            const DevObject& ow = objs[rw.obj_depth & 0xffffffu];
+            const int ncp = ow.type == MP_TYPE_GREGORY ? 20 : 16;
+            if (fent < ncp) {
+                pf = ldg4(ow.cps + (size_t)rw.pid * ncp + fent);
                 return true;
             }
         }
         return false;
     };
     if constexpr (kPair) {
-        // a: the current range, b: the one after it in the schedule
-        uint32_t a = r, b = r, b_end = r_end;
-        if (a < total) next(b, b_end);
-        else b = total;
+        // the window: w0 the current range, w1 and w2 the next two in the
+        // schedule (total: none), e2 the end of w2's chunk
+        uint32_t w0 = r, w1 = total, w2 = total, e2 = r_end;
+        if (w0 < total) {
+            w1 = w0;
+            next(w1, e2);
+        }
+        if (w1 < total) {
+            w2 = w1;
+            next(w2, e2);
+        }
         float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (fetch_two(a, b, pf)) s_cq[0][lane >> 5][lane & 31] = pf;
+        if (fetch3(w0, w1, w2, pf)) s_cq[0][fslot][fent] = pf;
         __syncthreads();
         int qb = 0;
-        while (a < total) {
-            const RangeRec ra = canon[a];
+        while (w0 < total) {
+            const RangeRec ra = canon[w0];
             const uint32_t oa = ra.obj_depth & 0xffffffu;
             const DevObject& ob = objs[oa];
             const DevView& vw = views[ob.view];
-            RangeRec rb = ra;
-            bool two = false;
-            if (b < total) {
-                rb = canon[b];
-                two = (rb.obj_depth & 0xffffffu) == oa;
+            RangeRec rb = ra, rc = ra;
+            int k = 1;   // consecutive ranges of this object diced together
+            if (w1 < total) {
+                rb = canon[w1];
+                if ((rb.obj_depth & 0xffffffu) == oa) {
+                    k = 2;
+                    if (w2 < total) {
+                        rc = canon[w2];
+                        if ((rc.obj_depth & 0xffffffu) == oa) k = 3;
+                    }
+                }
             }
-            // the next window: c after this one or two, d after c; their
-            // control points load now and go to the other buffer after the dice
-            uint32_t c = b, c_end = b_end;
-            if (two && c < total) next(c, c_end);
-            uint32_t d = c, d_end = c_end;
-            if (c < total) next(d, d_end);
-            else d = total;
-            const bool have_pf = fetch_two(c, d, pf);
-            dice_pair<N>(s_cq[qb][0], s_cq[qb][1], ra, rb, two, ob, vw, s_pos, s_pxl, s_dep);
+            // the next window (the ranges after the k diced now); its control
+            // points load now and go to the other buffer after the dice
+            uint32_t x = w2;
+            auto adv = [&]() {
+                if (x < total) next(x, e2);
+                else x = total;
+                return x;
+            };
+            const uint32_t m3 = adv();
+            const uint32_t m4 = k >= 2 ? adv() : total;
+            const uint32_t m5 = k == 3 ? adv() : total;
+            const uint32_t n0 = k == 1 ? w1 : k == 2 ? w2 : m3;
+            const uint32_t n1 = k == 1 ? w2 : k == 2 ? m3 : m4;
+            const uint32_t n2 = k == 1 ? m3 : k == 2 ? m4 : m5;
+            const bool have_pf = fetch3(n0, n1, n2, pf);
+            dice_group<N>(s_cq[qb][0], s_cq[qb][1], s_cq[qb][2], ra, rb, rc, k, ob, vw, s_pos, s_pxl, s_dep);
             __syncthreads();
-            if (have_pf) s_cq[qb ^ 1][lane >> 5][lane & 31] = pf;
-            // one shading code path for both ranges (instruction cache)
+            if (have_pf) s_cq[qb ^ 1][fslot][fent] = pf;
+            // one shading code path for the ranges (instruction cache)
 #pragma nounroll
-            for (int k = 0; k <= (int)two; ++k)
-                shade_sample<N>(s_pos + k * G, s_pxl + k * G, s_dep + k * G, lane & 7, lane >> 3, lane, vw, prm,
-                                k ? b : a, quad_store, slow, natom);
-            a = c;
-            b = d;
-            b_end = d_end;
+            for (int j = 0; j < k; ++j)
+                shade_sample<N>(s_pos + j * G, s_pxl + j * G, s_dep + j * G, lane & 7, lane >> 3, lane, vw, prm,
+                                j == 0 ? w0 : j == 1 ? w1 : w2, quad_store, slow, natom);
+            w0 = n0;
+            w1 = n1;
+            w2 = n2;
             qb ^= 1;
             __syncthreads();
         }
