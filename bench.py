@@ -37,6 +37,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MI355X_CUS = 256          # compute units
+MI355X_CLOCK_HZ = 2.4e9   # peak engine clock
 TUNING_KEYS = ("split_engine", "work_sharing", "split_levels", "split_budget", "split_waves", "raster_per_cu",
                "raster_chunk")
 
@@ -369,6 +371,17 @@ def main():
     issue = None
     prof_name = "profiles/k_raster16_c2_torus_1080p.md"
     prof = os.path.join(ROOT, prof_name)
+    frame_prof = "profiles/frame_kernels_c2_torus_1080p.md"
+    if os.path.exists(os.path.join(ROOT, frame_prof)):
+        for ln in open(os.path.join(ROOT, frame_prof)):
+            if ln.startswith("frame_valu_wave_instructions:"):
+                # the frame's compute floor: its kernels' VALU wave
+                # instructions at one per CU per cycle (256 CUs, 2.4 GHz)
+                n_valu = int(ln.split(":")[1])
+                t_valu = n_valu / (MI355X_CUS * MI355X_CLOCK_HZ)
+                fr["valu"] = {"wave_instructions": n_valu, "issue_rate_per_s": MI355X_CUS * MI355X_CLOCK_HZ,
+                              "t_valu_us": round(t_valu * 1e6, 2), "frac": round(t_valu / (ms_wall * 1e-3), 4),
+                              "source": frame_prof + " (rocprofv3 SQ_INSTS_VALU of the frame's kernels)"}
     if os.path.exists(prof):
         for ln in open(prof):
             if ln.startswith("traffic_bytes_per_launch:"):
