@@ -1042,7 +1042,7 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
     // ranges of a two-level step: five candidate decisions each on one lane
     // pass (a wider two-level step costs more sample passes than the level
     // it saves)
-    constexpr int SPEC = NT / 5 < 128 ? NT / 5 : 128;
+    constexpr int SPEC = NT / 5;
     constexpr int XCH = (kSpecSamples * SPEC > 9 * STEP ? kSpecSamples * SPEC : 9 * STEP) * 5;
     static_assert(5 * SPEC <= NT, "two-level candidates");
     constexpr int DBUF = NT;                       // buffered DRAW ranges
