@@ -165,8 +165,8 @@ def test_backward_and_readback_check_buffer_sizes():
 
 
 def test_adjoint_at_the_c5_size(oracle):
-    """BASELINE configuration 5 at its stated size: 32 views of the displaced
-    torus at 960x540, loss = L2 to a target rendered with control points
+    """The forward + adjoint workload at its full size: 32 views of the
+    displaced torus at 960x540, loss = L2 to a target rendered with control points
     + N(0, 0.01), amplitude 0.025 and albedo + 0.1.  The GPU gradients w.r.t.
     all 4096x16x3 control points, the amplitude and the albedo against the
     FP64 oracle over all 32 views (<= 1e-3 relative), and the per-view
