@@ -2969,8 +2969,10 @@ __global__ __launch_bounds__(256) void k_resolve(const DevView* __restrict__ vie
         MP_GLOBAL u64x2* kp = gptr(reinterpret_cast<u64x2*>(vw.keys + i));
         const u64x2 ka = kp[0], kb = kp[1];
         const u64x2 empty = { MP_EMPTY_KEY, MP_EMPTY_KEY };
-        kp[0] = empty;
-        kp[1] = empty;
+        // only key pairs a sample wrote are cleared (most of a frame's keys
+        // stay empty: their writes would be most of the kernel's traffic)
+        if ((ka.x & ka.y) != MP_EMPTY_KEY) kp[0] = empty;
+        if ((kb.x & kb.y) != MP_EMPTY_KEY) kp[1] = empty;
         const unsigned long long key[4] = { ka.x, ka.y, kb.x, kb.y };
         const uint32_t t = i >> 6, l = i & 63;
         const int X = (int)(t % (uint32_t)vw.grid_w) * 8 + (int)(l & 7);
