@@ -1010,7 +1010,7 @@ static int launch_back_end(mp_context* ctx, hipEvent_t* ev = nullptr)
 {
     CanonBuffers cb = canon_buffers(ctx);
     if (ctx->frame_ranked)
-        HIPCHK(ctx, launch_place_ranked(ctx->d_objs, ctx->d_views, ctx->n_views, cb, ctx->d_fc, ctx->num_cus,
+        HIPCHK(ctx, launch_place_ranked(ctx->d_views, ctx->n_views, cb, ctx->d_fc, ctx->num_cus,
                                         ctx->stream));
     else
         HIPCHK(ctx, launch_canonicalize(ctx->d_objs, ctx->d_views, ctx->n_views, cb, ctx->d_fc,

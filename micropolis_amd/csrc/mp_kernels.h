@@ -171,7 +171,8 @@ int split_persistent_occupancy(size_t lds_bytes, int waves);   // waves: 2 or 3 
 // patch_types: bit 0 some object has Bezier patches, bit 1 some has Gregory
 // patches (the kernel build without the other evaluation is launched)
 // ranked: the groups also rank their DRAW ranges inside each patch
-// (cb.rank) and write every root's DRAW count with a plain store, for
+// (cb.rank: the range's global patch index << 16 | its rank) and write
+// every root's DRAW count with a plain store, for
 // launch_place_ranked; a group with more DRAW ranges than its LDS key store
 // flags overflow bit 32 (the frame must then run unranked).  Unranked: the
 // counts are added atomically (zero on entry) for launch_canonicalize.
@@ -183,7 +184,7 @@ bool forest_rank_fits(uint32_t n_gpids);
 // The canonical order of a ranked forest frame in one launch: every
 // work-group scans the per-patch counts and places its share of the draw
 // list at offset(patch) + rank (no bucket pass, no rank kernel).
-hipError_t launch_place_ranked(const DevObject* objs, DevView* views, int n_views, const CanonBuffers& cb,
+hipError_t launch_place_ranked(DevView* views, int n_views, const CanonBuffers& cb,
                                FrameCounters* fc, int num_cus, hipStream_t s);
 // whether a frame's object and view tables fit the forest split's LDS copy
 bool split_forest_fits(int n_objs, int n_views);

@@ -1316,7 +1316,7 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
                     const uint32_t j0 = s_col[bin], j1 = s_cur[bin];
                     uint32_t r = j0 - s_col[sl * ncol];
                     for (uint32_t j = j0; j < j1; ++j) r += s_bk[j] < K ? 1u : 0u;
-                    cb.rank[(size_t)wid * kForestKeys + s_bg[p]] = r;
+                    cb.rank[(size_t)wid * kForestKeys + s_bg[p]] = (((uint32_t)wid + sl * gridDim.x) << 16) | r;
                 }
                 for (int k = tid; k < nroots; k += NT) {
                     const uint32_t a = s_col[(uint32_t)k * ncol];
@@ -1363,7 +1363,7 @@ __global__ __launch_bounds__(NT) void k_split_forest(const DevObject* __restrict
                         r += c;
                     }
                     for (; j < j1; ++j) r += s_bk[j] < K ? 1u : 0u;
-                    cb.rank[(size_t)wid * kForestKeys + s_bg[p]] = r;
+                    cb.rank[(size_t)wid * kForestKeys + s_bg[p]] = (((uint32_t)wid + sl * gridDim.x) << 16) | r;
                 }
                 for (int k = tid; k < nroots; k += NT)
                     cb.counts[(uint32_t)wid + (uint32_t)k * gridDim.x] = s_hist[k];
@@ -1808,8 +1808,10 @@ __global__ __launch_bounds__(1024) void k_scan_scatter(const DevObject* __restri
 // per-patch DRAW counts (kFusedScan at most, 16 per thread in LDS).
 constexpr int kPlaceThreads = 256;
 
-__global__ __launch_bounds__(kPlaceThreads) void k_place_ranked(const DevObject* __restrict__ objs,
-                                                                CanonBuffers cb, FrameCounters* fc, DevView* views,
+// the forest's rank words hold a global patch index and a rank, 16 bits each
+static_assert(kFusedScan <= 65536 && kForestKeys <= 65536, "patch index and rank share a word");
+
+__global__ __launch_bounds__(kPlaceThreads) void k_place_ranked(CanonBuffers cb, FrameCounters* fc, DevView* views,
                                                                 int n_views)
 {
     constexpr uint32_t PER = kFusedScan / kPlaceThreads;   // counts per thread
@@ -1819,6 +1821,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place_ranked(const DevObject*
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const uint32_t g = blockIdx.x;
     const uint32_t cnt = min(cb.forest_total[g], (uint32_t)kForestKeys);
+    const uint32_t ovf = fc->overflow;   // final: the split has ended
     // the group's records and ranks load now, in flight during the scan
     // (read only once the frame is known whole)
     constexpr uint32_t PR = (kForestKeys + kPlaceThreads - 1) / kPlaceThreads;
@@ -1877,7 +1880,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place_ranked(const DevObject*
     // counts that do not match its output, and a frame with more ranges than
     // the canonical list holds must grow it: nothing is placed (the host
     // renders the frame again)
-    const bool whole = (fc->overflow & 34u) == 0u && counted <= cb.cap;
+    const bool whole = (ovf & 34u) == 0u && counted <= cb.cap;
     if (g == 0) {
         if (t == 0) {
             fc->total_ranges = whole ? counted : 0u;
@@ -1894,8 +1897,8 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place_ranked(const DevObject*
 #pragma unroll
     for (uint32_t j = 0; j < PR; ++j) {
         if (t + j * kPlaceThreads < cnt) {
-            // gpid_of: the object's first global patch index + pid
-            const uint32_t slot = s_off[objs[w[j][1] & 0xffffffu].gpid_base + w[j][0]] + rk[j];
+            // rk: the range's global patch index << 16 | its rank in the patch
+            const uint32_t slot = s_off[rk[j] >> 16] + (rk[j] & 0xffffu);
             if (slot < cb.cap) {
                 uint32_t* dst = reinterpret_cast<uint32_t*>(cb.canon + slot);
 #pragma unroll
@@ -1905,14 +1908,13 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place_ranked(const DevObject*
     }
 }
 
-hipError_t launch_place_ranked(const DevObject* objs, DevView* views, int n_views, const CanonBuffers& cb,
+hipError_t launch_place_ranked(DevView* views, int n_views, const CanonBuffers& cb,
                                FrameCounters* fc, int num_cus, hipStream_t s)
 {
     if (cb.n_gpids > kFusedScan || !cb.rank || !cb.forest_out || !cb.forest_total || !cb.forest_groups)
         return hipErrorInvalidValue;
     (void)num_cus;
-    hipLaunchKernelGGL(k_place_ranked, dim3(cb.forest_groups), dim3(kPlaceThreads), 0, s, objs, cb, fc, views,
-                       n_views);
+    hipLaunchKernelGGL(k_place_ranked, dim3(cb.forest_groups), dim3(kPlaceThreads), 0, s, cb, fc, views, n_views);
     return hipGetLastError();
 }
 
