@@ -81,6 +81,24 @@ def frame_roofline(st, patches, ncp, W, H, ms, atomic_rate):
             "frac": round(t_roof / (ms * 1e-3), 5)}
 
 
+def frame_valu(profile, ms):
+    """The frame's compute floor: its kernels' VALU wave instructions
+    (rocprofv3 SQ_INSTS_VALU of the profiling command, recorded in the profile
+    as 'frame_valu_wave_instructions: N'; not measured in this run) at one per
+    CU per cycle (256 CUs, 2.4 GHz), against a frame of ms milliseconds."""
+    path = os.path.join(ROOT, profile)
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.startswith("frame_valu_wave_instructions:"):
+            n = int(ln.split(":")[1])
+            t = n / (MI355X_CUS * MI355X_CLOCK_HZ)
+            return {"wave_instructions": n, "issue_rate_per_s": MI355X_CUS * MI355X_CLOCK_HZ,
+                    "t_valu_us": round(t * 1e6, 2), "frac": round(t / (ms * 1e-3), 4),
+                    "source": profile + " (rocprofv3 SQ_INSTS_VALU of the frame's kernels)"}
+    return None
+
+
 def measure_c1(atomic_rate):
     """The 64-patch sphere, BREADTH (host-driven level loop), API frames."""
     from micropolis_amd import Renderer, scenes
@@ -102,7 +120,7 @@ def measure_c1(atomic_rate):
             "frame_roofline": frame_roofline(st, w.patch_count, 16, W, H, wall, atomic_rate)}
 
 
-def measure_c4(steps, atomic_rate):
+def measure_c4(steps, atomic_rate, autotune=True):
     """16384 Gregory patches on 32 stacked planes: the multipass BOUNDED form
     (batches of reyes_patches_per_pass, a host sync per batch) against the
     persistent BALANCED work queue replayed as one hipGraph."""
@@ -114,7 +132,7 @@ def measure_c4(steps, atomic_rate):
     r = Renderer(**w.options)
     draw_frame(r, v)
     st = r.stats()
-    tuning = r.autotune(3)
+    tuning = r.autotune(3) if autotune else r.tuning()
     r.replay(1, use_graph=True)
     graph_ms = r.replay(steps, use_graph=True)
     api_wall, api_dev = api_frames(r, v, 5)
@@ -136,7 +154,8 @@ def measure_c4(steps, atomic_rate):
             "mmicropolys_per_s": round(st["micropolygons"] / (graph_ms * 1e-3) / 1e6, 3),
             "same_output": bool(np.array_equal(ids_b, ids_m)) and int(st["micropolygons"]) == int(stb["micropolygons"]),
             "tuning": {k: tuning[k] for k in TUNING_KEYS},
-            "frame_roofline": frame_roofline(st, w.patch_count, 20, W, H, graph_ms, atomic_rate)}
+            "frame_roofline": dict(frame_roofline(st, w.patch_count, 20, W, H, graph_ms, atomic_rate),
+                                   valu=frame_valu("profiles/frame_kernels_c4_gregory_16384p_1080p.md", graph_ms))}
 
 
 def measure_batch(steps, use_graph):
@@ -371,17 +390,9 @@ def main():
     issue = None
     prof_name = "profiles/k_raster16_c2_torus_1080p.md"
     prof = os.path.join(ROOT, prof_name)
-    frame_prof = "profiles/frame_kernels_c2_torus_1080p.md"
-    if os.path.exists(os.path.join(ROOT, frame_prof)):
-        for ln in open(os.path.join(ROOT, frame_prof)):
-            if ln.startswith("frame_valu_wave_instructions:"):
-                # the frame's compute floor: its kernels' VALU wave
-                # instructions at one per CU per cycle (256 CUs, 2.4 GHz)
-                n_valu = int(ln.split(":")[1])
-                t_valu = n_valu / (MI355X_CUS * MI355X_CLOCK_HZ)
-                fr["valu"] = {"wave_instructions": n_valu, "issue_rate_per_s": MI355X_CUS * MI355X_CLOCK_HZ,
-                              "t_valu_us": round(t_valu * 1e6, 2), "frac": round(t_valu / (ms_wall * 1e-3), 4),
-                              "source": frame_prof + " (rocprofv3 SQ_INSTS_VALU of the frame's kernels)"}
+    valu = frame_valu("profiles/frame_kernels_c2_torus_1080p.md", ms_wall)
+    if valu:
+        fr["valu"] = valu
     if os.path.exists(prof):
         for ln in open(prof):
             if ln.startswith("traffic_bytes_per_launch:"):
@@ -396,7 +407,7 @@ def main():
     configs = {}
     if not args.no_configs:
         configs["c1"] = measure_c1(atomic_rate)
-        configs["c4"] = measure_c4(max(3, min(args.steps, 10)), atomic_rate)
+        configs["c4"] = measure_c4(max(3, min(args.steps, 10)), atomic_rate, not args.no_autotune)
     batch = None if args.no_batch else measure_batch(max(3, min(args.steps, 10)), use_graph)
     adjoint = None if args.no_adjoint else run_adjoint_child(max(3, min(args.steps, 10)))
     cpu = None if args.no_cpu_baseline else measure_cpu_baseline(w, mpolys)
